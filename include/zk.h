@@ -1,0 +1,139 @@
+/*
+ * zk.h -- C ABI of libzk.so, the MI355X (gfx950) explicit 2D Euler update.
+ *
+ * One handle owns one whole-domain state on one GPU: the conserved variables
+ * as structure-of-arrays planes in HBM, one HIP stream, and the scratch the
+ * stage kernels need.  A handle is not thread-safe; different handles are
+ * independent.  Every call returns ZK_OK (0) or a nonzero status, with the
+ * message available from zk_last_error().  Host arrays passed in are
+ * caller-owned and copied; nothing is retained after the call returns.
+ *
+ * Array convention: a field is a row-major ny*nx plane of doubles, row i is
+ * the i-th row of the input .dat files (y downward) and column j is x -- the
+ * order `file >> mat(i,j)` reads them in src_WENO/fluid.cpp:33-40.  A state
+ * is 4*nm planes ordered (variable c, mode m) with nm = 1 for WENO and
+ * (P+1)^2 for DG, mode m = px*(P+1)+py as src_DG/fluid.cpp:63-65.
+ *
+ * The reference is two executables (solver_WENO, solver_DG2/3/4) whose hot
+ * call is `updateMesh(Mesh&, double dt, int rank, int num_procs)`; each entry
+ * point below names the reference interface it replaces.
+ */
+#ifndef ZK_H
+#define ZK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zk_handle zk_handle;
+typedef int zk_status;
+
+#define ZK_OK 0
+#define ZK_ERR_ARG 1      /* bad argument or call order */
+#define ZK_ERR_HIP 2      /* HIP runtime error */
+#define ZK_ERR_NUMERIC 3  /* non-finite value detected */
+#define ZK_ERR_CFL 4      /* DG CFL limit exceeded (src_DG/solver.cpp:245-260) */
+
+/* method */
+#define ZK_METHOD_WENO 0  /* src_WENO: WENO5 FV + HLLC, SSP-RK3 */
+#define ZK_METHOD_DG 1    /* src_DG: modal DG P=1..3, SSP-RK3 */
+/* precision of the WENO bulk arithmetic (state and RK accumulation stay FP64) */
+#define ZK_PREC_FP64 0
+#define ZK_PREC_FP32_BULK 1
+/* numerical flux */
+#define ZK_FLUX_HLLC 0    /* src_WENO/fluid.cpp:268-317 / src_DG/fluid.cpp:450-520 */
+#define ZK_FLUX_HLL 1     /* hlleFlux, src_DG/fluid.cpp:525-580 */
+/* DG limiter */
+#define ZK_LIMITER_NONE 0
+#define ZK_LIMITER_PERSSON_ZS 1 /* applyLimiter, src_DG/fluid.cpp:824-992 */
+#define ZK_LIMITER_TVB 2        /* Cockburn-Shu TVB (M) + Zhang-Shu positivity */
+#define ZK_LIMITER_LEARNED 3    /* MLP troubled-cell indicator gating the TVB limiter */
+
+typedef struct zk_config {
+    int method;      /* ZK_METHOD_* */
+    int P;           /* DG degree 1..3 (DG_P_VAL, src_DG/fluid.h:35-39); ignored for WENO */
+    int precision;   /* ZK_PREC_* */
+    int flux;        /* ZK_FLUX_* */
+    int limiter;     /* ZK_LIMITER_*; DG only */
+    double tvb_M;    /* TVB constant M (north_star: 50) */
+    int device;      /* HIP device ordinal */
+} zk_config;
+
+/* Create a handle.  Replaces the `Mesh` object's lifetime
+ * (src_WENO/fluid.h:26-80, src_DG/fluid.h:44-110). */
+zk_status zk_create(const zk_config *cfg, zk_handle **out);
+
+/* Release all device memory and the stream. */
+zk_status zk_destroy(zk_handle *h);
+
+/* Message of the last failed call on h (or of zk_create when h is NULL). */
+const char *zk_last_error(const zk_handle *h);
+
+/* Load a primitive-variable grid and convert to conserved variables; DG
+ * modes >= 1 start at zero.  Replaces Mesh(folderPath) without the file
+ * reading (src_WENO/fluid.cpp:68-94; src_DG/fluid.cpp:216-244 with
+ * initDGModes :173-182).  bctype codes: 0 updated, -1 copy ghost, 1 slip
+ * ghost (DG), -2 periodic ghost (extension). */
+zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma,
+                  const double *rho, const double *u, const double *v,
+                  const double *p, const int32_t *bctype);
+
+/* Overwrite the whole state (4*nm planes), e.g. L2-projected DG modes or a
+ * restore.  No reference counterpart (the reference cannot resume). */
+zk_status zk_set_state(zk_handle *h, const double *Q);
+
+/* Copy the whole state (4*nm planes) to the host. */
+zk_status zk_get_state(zk_handle *h, double *Q);
+
+/* Copy the cell averages U0..U3 (mode 0) into four caller-owned planes.
+ * Replaces syncCellAverages + the field part of saveMeshData
+ * (src_DG/fluid.cpp:190-196, src_WENO/fluid.cpp:558-576). */
+zk_status zk_get_conserved(zk_handle *h, double *U0, double *U1, double *U2, double *U3);
+
+/* Maximum wave speed.  WENO: computeMaxSpeed over every cell
+ * (src_WENO/fluid.cpp:539-553).  DG: computeMaxSpeedDG_omp at the four cell
+ * corners of bctype-0 cells (src_DG/solver.cpp:26-64). */
+zk_status zk_max_speed(zk_handle *h, double *out);
+
+/* Advance nsteps SSP-RK3 steps with a fixed dt.  Replaces nsteps calls of
+ * updateMesh(mesh, dt, rank, nprocs) (src_WENO/fluid.cpp:468-493,
+ * src_DG/fluid.cpp:1005-1054).  Asynchronous: returns after enqueueing. */
+zk_status zk_step(zk_handle *h, double dt, int nsteps);
+
+/* Advance nsteps steps with dt = cfl * da / lambda_max recomputed on the
+ * device before every step (lambda from zk_max_speed's rule), clipped so that
+ * time never passes t_end; stops early at t_end.  The time and step count
+ * reached are returned.  The CFL-correct replacement of the `auto`/`time`
+ * dt rule of src_WENO/solver.cpp:148-171, which sets dt = lambda * h. */
+zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps,
+                      double *t_out, int *steps_out);
+
+/* Wait for all enqueued work on the handle's stream. */
+zk_status zk_sync(zk_handle *h);
+
+/* Time nsteps fixed-dt steps with HIP events recorded on the handle's own
+ * stream; *ms receives the elapsed milliseconds (measurement helper). */
+zk_status zk_step_timed(zk_handle *h, double dt, int nsteps, float *ms);
+
+/* Per-cell limiter flags of the last stage (ny*nx bytes; TVB / learned
+ * limiter: 1 = limited).  No reference counterpart. */
+zk_status zk_get_flags(zk_handle *h, uint8_t *flags);
+
+/* Kernel timing (measurement helper, no reference counterpart): while
+ * enabled, every kernel launch is bracketed by a pair of HIP events on the
+ * handle's stream.  zk_profile(h, 1) clears earlier records and starts;
+ * zk_profile_report writes a JSON object {"kernel": [launches, total_ms]}. */
+zk_status zk_profile(zk_handle *h, int enable);
+zk_status zk_profile_report(zk_handle *h, char *buf, size_t len);
+
+/* Count of cells updated per stage (bctype 0 inside [3,ny-3)x[3,nx-3),
+ * the loop domain of src_WENO/fluid.cpp:398-402). */
+zk_status zk_updated_cells(zk_handle *h, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZK_H */

@@ -1,0 +1,630 @@
+// dg_kernels.hip -- modal DG (P = 1..3) stage kernels for gfx950.
+//
+// A stage is three launches over the update range [3,ny-3)x[3,nx-3):
+//   1. dg_face_kernel<P, X>: one thread per x face, evaluates both traces at
+//      the NQ = P+1 Gauss points of the face, computes the numerical flux once
+//      per point and stores its projection g[c][p] = sum_q w_q L_p(s_q) F*_q[c]
+//      (4*(P+1) doubles per face).  The reference evaluates every face twice,
+//      once from each cell (src_DG/fluid.cpp:731-796); both evaluations see the
+//      same inputs, so one evaluation gives the same flux.
+//   2. the same for y faces (flux called with rho*u and rho*v swapped).
+//   3. dg_cell_kernel<P>: one thread per cell: volume integral at NQ x NQ
+//      Gauss points (src_DG/fluid.cpp:698-718), the four face terms from g
+//      (phi_m(+-1, s) = (+-1)^px L_py(s) makes each face term a signed copy of
+//      g), the invMass(m) * 0.5/h scale (:800-804), the Shu-Osher combination
+//      (:1022-1053) and, for the reference limiter, the Persson filter and the
+//      positivity scaling (:824-992) of the freshly combined cell.
+// The TVB limiter needs the neighbours' new means, so it runs as its own
+// kernel after the combination, followed by the positivity scaling.
+#include "zk_internal.h"
+#include "zk_device.h"
+
+namespace zk {
+
+namespace {
+
+constexpr int DG_MAXNQ = 4;
+constexpr int DG_MAXNM = 16;
+
+// Basis tables, filled on the host in plain double arithmetic with the same
+// products the reference forms in phi2/dphi2 (src_DG/fluid.cpp:67-76).
+struct DgTables {
+    double w[DG_MAXNQ];                              // GL weights
+    double wl[DG_MAXNQ][DG_MAXNQ];                   // w_q * L_p(s_q)  [p][q]
+    double sgn[DG_MAXNQ];                            // L_p(-1) = (-1)^p
+    double tr_x[2][DG_MAXNQ][DG_MAXNM];              // phi_m(-1 / +1, eta_q)
+    double tr_y[2][DG_MAXNQ][DG_MAXNM];              // phi_m(xi_q, -1 / +1)
+    double phi[DG_MAXNQ][DG_MAXNQ][DG_MAXNM];        // phi_m(xi_a, eta_b)
+    double dxi[DG_MAXNQ][DG_MAXNQ][DG_MAXNM];        // d phi_m / d xi
+    double det[DG_MAXNQ][DG_MAXNQ][DG_MAXNM];        // d phi_m / d eta
+    double ww[DG_MAXNQ][DG_MAXNQ];                   // w_a * w_b
+    double corner[4][DG_MAXNM];                      // phi_m at (+-1, +-1) for the CFL speed
+    double zs[4][4][DG_MAXNM];                       // truncated basis() at the 4x4 ZS grid
+    double invm[DG_MAXNM];                           // (2px+1)(2py+1)
+};
+
+__constant__ DgTables c_tab[3];  // index P-1
+
+double legP_h(int n, double x)
+{
+    switch (n) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return 0.5 * (3.0 * x * x - 1.0);
+    case 3: return 0.5 * (5.0 * x * x * x - 3.0 * x);
+    default: return 0.0;
+    }
+}
+double legD_h(int n, double x)
+{
+    switch (n) {
+    case 0: return 0.0;
+    case 1: return 1.0;
+    case 2: return 3.0 * x;
+    case 3: return 0.5 * (15.0 * x * x - 3.0);
+    default: return 0.0;
+    }
+}
+double zsP_h(int p, double x)
+{
+    if (p == 0) return 1.0;
+    if (p == 1) return x;
+    if (p == 2) return 0.5 * (3.0 * x * x - 1.0);
+    return 0.0;
+}
+
+// Gauss-Legendre literals of src_DG/fluid.cpp:94-111.
+const double GLP_H[5][4] = {{0, 0, 0, 0},
+                            {0.0, 0, 0, 0},
+                            {-0.5773502691896257, 0.5773502691896257, 0, 0},
+                            {-0.7745966692414834, 0.0, 0.7745966692414834, 0},
+                            {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526}};
+const double GLW_H[5][4] = {{0, 0, 0, 0},
+                            {2.0, 0, 0, 0},
+                            {1.0, 1.0, 0, 0},
+                            {0.5555555555555556, 0.8888888888888888, 0.5555555555555556, 0},
+                            {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538}};
+
+DgTables make_tables(int P)
+{
+    DgTables T{};
+    const int NQ = P + 1, NM = NQ * NQ;
+    const double *gp = GLP_H[NQ], *gw = GLW_H[NQ];
+    const double zsp[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+    for (int q = 0; q < NQ; ++q) {
+        T.w[q] = gw[q];
+        for (int p = 0; p < NQ; ++p) T.wl[p][q] = gw[q] * legP_h(p, gp[q]);
+    }
+    for (int p = 0; p < NQ; ++p) T.sgn[p] = legP_h(p, -1.0);
+    const double XI[4] = {-1.0, +1.0, -1.0, +1.0}, ETA[4] = {-1.0, -1.0, +1.0, +1.0};
+    for (int m = 0; m < NM; ++m) {
+        const int px = m / NQ, py = m % NQ;
+        T.invm[m] = (double)((2 * px + 1) * (2 * py + 1));
+        for (int q = 0; q < NQ; ++q) {
+            T.tr_x[0][q][m] = legP_h(px, -1.0) * legP_h(py, gp[q]);
+            T.tr_x[1][q][m] = legP_h(px, +1.0) * legP_h(py, gp[q]);
+            T.tr_y[0][q][m] = legP_h(px, gp[q]) * legP_h(py, -1.0);
+            T.tr_y[1][q][m] = legP_h(px, gp[q]) * legP_h(py, +1.0);
+        }
+        for (int a = 0; a < NQ; ++a)
+            for (int b = 0; b < NQ; ++b) {
+                T.phi[a][b][m] = legP_h(px, gp[a]) * legP_h(py, gp[b]);
+                T.dxi[a][b][m] = legD_h(px, gp[a]) * legP_h(py, gp[b]);
+                T.det[a][b][m] = legP_h(px, gp[a]) * legD_h(py, gp[b]);
+            }
+        for (int k = 0; k < 4; ++k) T.corner[k][m] = legP_h(px, XI[k]) * legP_h(py, ETA[k]);
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) T.zs[a][b][m] = zsP_h(px, zsp[a]) * zsP_h(py, zsp[b]);
+    }
+    for (int a = 0; a < NQ; ++a)
+        for (int b = 0; b < NQ; ++b) T.ww[a][b] = gw[a] * gw[b];
+    return T;
+}
+
+// Euler flux pair with rho and p floored at 1e-3 (eulerFluxX/Y,
+// src_DG/fluid.cpp:123-143); both share the primitive recovery.
+__device__ __forceinline__ void euler_fg(const double U[4], double gam, double F[4], double G[4])
+{
+    constexpr double eps = 1e-3;
+    const double rho = smax(U[0], eps);
+    const double u = U[1] / rho, v = U[2] / rho;
+    const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
+    F[0] = rho * u; F[1] = rho * u * u + p; F[2] = rho * u * v; F[3] = (U[3] + p) * u;
+    G[0] = rho * v; G[1] = rho * u * v; G[2] = rho * v * v + p; G[3] = (U[3] + p) * v;
+}
+
+template <int P>
+__device__ __forceinline__ void load_cell(const double *__restrict__ Q, const int32_t *__restrict__ src,
+                                          size_t np, size_t k, double A[4][(P + 1) * (P + 1)])
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    const int32_t e = src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) A[c][m] = Q[(size_t)(c * NM + m) * np + s];
+    if (e & 1)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) A[1][m] = -A[1][m];
+    if (e & 2)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) A[2][m] = -A[2][m];
+}
+
+struct DgArgs {
+    const double *Qn;
+    const double *Qin;
+    double *Qout;
+    const int32_t *src;
+    const uint8_t *upd;
+    const int32_t *bct;
+    const StepScalars *S;
+    double *gx, *gy;
+    int ny, nx, nxu, nyu;
+    size_t np, nfx, nfy;
+    double da, gamma;
+    double ca, cb;
+    int stage, flux, limiter;
+};
+
+// One thread per face.  X: faces between (i, j-1) and (i, j), j in [3, nx-3];
+// otherwise faces between (i-1, j) and (i, j), i in [3, ny-3].
+template <int P, bool X>
+__global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const DgTables &T = c_tab[P - 1];
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nf = X ? A.nfx : A.nfy;
+    if (f >= nf || A.S->done) return;
+    int i, j;
+    size_t kl, kr;
+    if (X) {
+        const int w = A.nxu + 1;
+        i = 3 + (int)(f / w); j = 3 + (int)(f % w);
+        kl = (size_t)i * A.nx + (j - 1); kr = (size_t)i * A.nx + j;
+    } else {
+        const int w = A.nxu;
+        i = 3 + (int)(f / w); j = 3 + (int)(f % w);
+        kl = (size_t)(i - 1) * A.nx + j; kr = (size_t)i * A.nx + j;
+    }
+    double AL[4][NM], AR[4][NM];
+    load_cell<P>(A.Qin, A.src, A.np, kl, AL);
+    load_cell<P>(A.Qin, A.src, A.np, kr, AR);
+
+    double g[4][NQ];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) g[c][p] = 0.0;
+
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double UL[4] = {0.0, 0.0, 0.0, 0.0}, UR[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const double bl = X ? T.tr_x[1][q][m] : T.tr_y[1][q][m];
+            const double br = X ? T.tr_x[0][q][m] : T.tr_y[0][q][m];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                UL[c] += AL[c][m] * bl;
+                UR[c] += AR[c][m] * br;
+            }
+        }
+        double Fs[4];
+        if (X) {
+            if (A.flux == ZK_FLUX_HLL) hll_f64(UL, UR, A.gamma, Fs);
+            else hllc_dg_f64(UL, UR, A.gamma, Fs);
+        } else {
+            const double a[4] = {UL[0], UL[2], UL[1], UL[3]}, b[4] = {UR[0], UR[2], UR[1], UR[3]};
+            double Gs[4];
+            if (A.flux == ZK_FLUX_HLL) hll_f64(a, b, A.gamma, Gs);
+            else hllc_dg_f64(a, b, A.gamma, Gs);
+            Fs[0] = Gs[0]; Fs[1] = Gs[2]; Fs[2] = Gs[1]; Fs[3] = Gs[3];
+        }
+#pragma unroll
+        for (int p = 0; p < NQ; ++p)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) g[c][p] += T.wl[p][q] * Fs[c];
+    }
+    double *G = X ? A.gx : A.gy;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) G[(size_t)(c * NQ + p) * nf + f] = g[c][p];
+}
+
+// Positivity scaling of one cell (src_DG/fluid.cpp:897-991).
+template <int P>
+__device__ __forceinline__ void zs_cell(double Q[4][(P + 1) * (P + 1)], double gamma)
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    const DgTables &T = c_tab[P - 1];
+    constexpr double RHO_MIN = 1e-5, P_MIN = 1e-5, EPS = 1e-5;
+    Q[0][0] = smax(Q[0][0], RHO_MIN);
+    const double rho0 = Q[0][0], mx0 = Q[1][0], my0 = Q[2][0];
+    double E0 = Q[3][0];
+    const double p0 = (gamma - 1.0) * (E0 - 0.5 * (mx0 * mx0 + my0 * my0) / rho0);
+    if (p0 < P_MIN) {
+        E0 = P_MIN / (gamma - 1.0) + 0.5 * (mx0 * mx0 + my0 * my0) / rho0;
+        Q[3][0] = E0;
+    }
+    double theta_rho = 1.0;
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+            double rho = 0.0;
+#pragma unroll
+            for (int m = 0; m < NM; m++) rho += Q[0][m] * T.zs[a][b][m];
+            if (rho < RHO_MIN) {
+                const double t = (rho0 - RHO_MIN) / (rho0 - rho + EPS);
+                theta_rho = smin(theta_rho, t);
+            }
+        }
+    if (theta_rho < 1.0)
+#pragma unroll
+        for (int m = 1; m < NM; m++)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Q[c][m] *= theta_rho;
+
+    double theta_p = 1.0;
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+            double rho = 0.0, mx = 0.0, my = 0.0, E = 0.0;
+#pragma unroll
+            for (int m = 0; m < NM; m++) {
+                const double phi = T.zs[a][b][m];
+                rho += Q[0][m] * phi;
+                mx += Q[1][m] * phi;
+                my += Q[2][m] * phi;
+                E += Q[3][m] * phi;
+            }
+            if (rho <= RHO_MIN) continue;
+            const double u = mx / rho, v = my / rho;
+            const double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+            if (p < P_MIN) {
+                const double p_avg = (gamma - 1.0) * (Q[3][0] - 0.5 * (Q[1][0] * Q[1][0] + Q[2][0] * Q[2][0]) / Q[0][0]);
+                const double t = (p_avg - P_MIN) / (p_avg - p + EPS);
+                theta_p = smin(theta_p, t);
+            }
+        }
+    if (theta_p < 1.0)
+#pragma unroll
+        for (int m = 1; m < NM; m++)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Q[c][m] *= theta_p;
+}
+
+// Persson modal-decay filter of one cell (src_DG/fluid.cpp:841-887).
+template <int P>
+__device__ __forceinline__ void persson_cell(double Q[4][(P + 1) * (P + 1)])
+{
+    constexpr double kappa = 1.0, EPS = 1e-5;
+    // s0 = -4 log10(P) for P < 3, -3 log10(P) for P >= 3, evaluated as the
+    // reference does, with the device log10.
+    const double s0 = (P >= 3) ? -3.0 * log10((double)P) : -4.0 * log10((double)P);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        double Etot = 0.0, Ep = 0.0;
+#pragma unroll
+        for (int px = 0; px <= P; px++)
+#pragma unroll
+            for (int py = 0; py <= P; py++) {
+                const double a = Q[c][px * (P + 1) + py];
+                Etot += a * a;
+                if (px == P || py == P) Ep += a * a;
+            }
+        const double s = log10(Ep / (Etot + EPS));
+        if (s > s0) {
+            double eps;
+            if (s > s0 + kappa) eps = 1.0;
+            else {
+                const double r = (s - s0) / kappa;
+                eps = 0.5 * (1.0 + sin(M_PI * (r - 0.5)));
+            }
+#pragma unroll
+            for (int px = 0; px <= P; px++)
+#pragma unroll
+                for (int py = 0; py <= P; py++)
+                    if (px == P || py == P) Q[c][px * (P + 1) + py] *= (1.0 - eps);
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const DgTables &T = c_tab[P - 1];
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)A.nxu * A.nyu || A.S->done) return;
+    const int ii = (int)(t / A.nxu), jj = (int)(t % A.nxu);
+    const int i = ii + 3, j = jj + 3;
+    const size_t k = (size_t)i * A.nx + j;
+    const size_t np = A.np;
+    const bool upd = A.upd[k] != 0;
+
+    double Q[4][NM];   // filled input of the cell, then its output
+    load_cell<P>(A.Qin, A.src, np, k, Q);
+    double rhs[4][NM];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) rhs[c][m] = 0.0;
+
+    if (upd) {
+        // volume term
+#pragma unroll
+        for (int a = 0; a < NQ; ++a)
+#pragma unroll
+            for (int b = 0; b < NQ; ++b) {
+                double U[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) U[c] += Q[c][m] * T.phi[a][b][m];
+                double F[4], G[4];
+                euler_fg(U, A.gamma, F, G);
+                const double ww = T.ww[a][b];
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    const double dx = T.dxi[a][b][m], dy = T.det[a][b][m];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) rhs[c][m] += ww * (F[c] * dx + G[c] * dy);
+                }
+            }
+        // face terms: right (x face j+1), left (x face j), down (y face i+1), up (y face i)
+        const size_t fr = (size_t)ii * (A.nxu + 1) + (jj + 1), fl = fr - 1;
+        const size_t fd = (size_t)(ii + 1) * A.nxu + jj, fu = (size_t)ii * A.nxu + jj;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                const double gR = A.gx[(size_t)(c * NQ + p) * A.nfx + fr];
+                const double gL = A.gx[(size_t)(c * NQ + p) * A.nfx + fl];
+                const double gD = A.gy[(size_t)(c * NQ + p) * A.nfy + fd];
+                const double gU = A.gy[(size_t)(c * NQ + p) * A.nfy + fu];
+#pragma unroll
+                for (int o = 0; o < NQ; ++o) {
+                    // x faces project onto py (= p), y faces onto px (= p)
+                    rhs[c][o * NQ + p] += T.sgn[o] * gL - gR;
+                    rhs[c][p * NQ + o] += T.sgn[o] * gU - gD;
+                }
+            }
+    }
+    const double dt = A.S->dt;
+    const double inv2h = 0.5 / A.da;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const double dU = rhs[c][m] * (T.invm[m] * inv2h);
+            const double qn = A.Qn[(size_t)(c * NM + m) * np + k];
+            Q[c][m] = (A.stage == 0) ? qn + dt * dU : A.ca * qn + A.cb * (Q[c][m] + dt * dU);
+        }
+    if (A.limiter == ZK_LIMITER_PERSSON_ZS) {
+        if (A.bct[k] == 0) persson_cell<P>(Q);
+        zs_cell<P>(Q, A.gamma);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) A.Qout[(size_t)(c * NM + m) * np + k] = Q[c][m];
+}
+
+// Mode-0 planes of Q after the ghost fill, for the TVB neighbour means.
+template <int P>
+__global__ void dg_means_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
+                                double *__restrict__ M, const StepScalars *__restrict__ S)
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np || S->done) return;
+    const int32_t e = src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double v = Q[(size_t)(c * NM) * np + s];
+        if ((c == 1 && (e & 1)) || (c == 2 && (e & 2))) v = -v;
+        M[(size_t)c * np + k] = v;
+    }
+}
+
+__device__ __forceinline__ double minmod3(double a, double b, double c)
+{
+    if (a > 0.0 && b > 0.0 && c > 0.0) return fmin(a, fmin(b, c));
+    if (a < 0.0 && b < 0.0 && c < 0.0) return fmax(a, fmax(b, c));
+    return 0.0;
+}
+
+// Component-wise TVB limiter (Cockburn & Shu 2001, modified minmod with
+// constant M) then the positivity scaling, in place on the stage output.
+template <int P>
+__global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, const double *__restrict__ M,
+                                                     const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
+                                                     int ny, int nx, size_t np, double Mh2, double gamma,
+                                                     const StepScalars *__restrict__ S)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nxu * nyu || S->done) return;
+    const int i = 3 + (int)(t / nxu), j = 3 + (int)(t % nxu);
+    const size_t k = (size_t)i * nx + j;
+    double C[4][NM];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+    int flag = 0;
+    if (bct[k] == 0) {
+        constexpr int mxm = NQ, mym = 1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double *a0 = M + (size_t)c * np;
+            const double ub = a0[k];
+            const double ax = C[c][mxm], ay = C[c][mym];
+            const double axl = (fabs(ax) <= Mh2) ? ax : minmod3(ax, a0[k + 1] - ub, ub - a0[k - 1]);
+            const double ayl = (fabs(ay) <= Mh2) ? ay : minmod3(ay, a0[k + nx] - ub, ub - a0[k - nx]);
+            if (axl != ax || ayl != ay) {
+#pragma unroll
+                for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
+                C[c][mxm] = axl;
+                C[c][mym] = ayl;
+                flag = 1;
+            }
+        }
+    }
+    zs_cell<P>(C, gamma);
+    flags[k] = (uint8_t)flag;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+}
+
+// max(|u| + c) at the four corners of every bctype-0 cell, floors 1e-12
+// (computeMaxSpeedDG_omp, src_DG/solver.cpp:26-64).
+template <int P>
+__global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restrict__ Q,
+                                                           const int32_t *__restrict__ bct, size_t np,
+                                                           double gam, unsigned long long *__restrict__ out)
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    const DgTables &T = c_tab[P - 1];
+    constexpr double eps = 1e-12;
+    double mx = 0.0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (size_t)gridDim.x * blockDim.x) {
+        if (bct[k] != 0) continue;
+        double A[4][NM];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int m = 0; m < NM; ++m) A[c][m] = Q[(size_t)(c * NM + m) * np + k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double U[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) U[c] += A[c][m] * T.corner[q][m];
+            const double rho = smax(U[0], eps);
+            const double u = U[1] / rho, v = U[2] / rho;
+            const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
+            const double a = sqrt(gam * p / rho);
+            mx = smax(mx, sqrt(u * u + v * v) + a);
+        }
+    }
+    block_max_to(mx, out);
+}
+
+__global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__restrict__ Qin,
+                               double *__restrict__ Qout, const int32_t *__restrict__ src,
+                               const int32_t *__restrict__ ring, int nring, size_t np, int nm, double ca,
+                               double cb, int stage, const StepScalars *__restrict__ S)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nring || S->done) return;
+    const size_t k = (size_t)ring[t];
+    const int32_t e = src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+    for (int p = 0; p < 4 * nm; ++p) {
+        const int var = p / nm;
+        const double qn = Qn[p * np + k];
+        double qf = Qin[p * np + s];
+        if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
+        Qout[p * np + k] = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
+    }
+}
+
+bool g_tables_ready[64] = {false};
+
+template <int P>
+void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, int stage)
+{
+    DgArgs A;
+    A.Qn = Qn; A.Qin = Qin; A.Qout = Qout;
+    A.src = H.src; A.upd = H.upd; A.bct = H.bct; A.S = H.scal;
+    A.gx = H.gx; A.gy = H.gy;
+    A.ny = H.g.ny; A.nx = H.g.nx; A.nxu = H.g.nx - 6; A.nyu = H.g.ny - 6;
+    A.np = H.g.np;
+    A.nfx = (size_t)(A.nxu + 1) * A.nyu;
+    A.nfy = (size_t)A.nxu * (A.nyu + 1);
+    A.da = H.g.da; A.gamma = H.g.gamma;
+    A.stage = stage;
+    A.ca = stage == 1 ? 0.75 : (1.0 / 3.0);
+    A.cb = stage == 1 ? 0.25 : (2.0 / 3.0);
+    A.flux = H.cfg.flux;
+    A.limiter = H.cfg.limiter;
+    const size_t ncell = (size_t)A.nxu * A.nyu;
+    if (ncell > 0) {
+        {
+            ProfScope ps(H, "dg_face_x");
+            hipLaunchKernelGGL((dg_face_kernel<P, true>), dim3((A.nfx + 127) / 128), dim3(128), 0, H.stream, A);
+        }
+        {
+            ProfScope ps(H, "dg_face_y");
+            hipLaunchKernelGGL((dg_face_kernel<P, false>), dim3((A.nfy + 127) / 128), dim3(128), 0, H.stream, A);
+        }
+        ProfScope ps(H, "dg_cell");
+        hipLaunchKernelGGL((dg_cell_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, A);
+    }
+    if (H.nring > 0) {
+        ProfScope ps(H, "dg_ring");
+        hipLaunchKernelGGL(dg_ring_kernel, dim3((H.nring + 255) / 256), dim3(256), 0, H.stream, Qn, Qin, Qout,
+                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal);
+    }
+    if (ncell > 0 && (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED)) {
+        {
+            ProfScope ps(H, "dg_means");
+            hipLaunchKernelGGL((dg_means_kernel<P>), dim3((H.g.np + 255) / 256), dim3(256), 0, H.stream, Qout, H.src,
+                           H.g.np, H.means, H.scal);
+        }
+        ProfScope ps(H, "dg_tvb");
+        hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
+                           H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.cfg.tvb_M * H.g.da * H.g.da, H.g.gamma,
+                           H.scal);
+    }
+}
+
+}  // namespace
+
+void dg_upload_tables(int device)
+{
+    if (device >= 0 && device < 64 && g_tables_ready[device]) return;
+    DgTables T[3] = {make_tables(1), make_tables(2), make_tables(3)};
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), T, sizeof(T), 0, hipMemcpyHostToDevice);
+    if (device >= 0 && device < 64) g_tables_ready[device] = true;
+}
+
+void dg_stage(Handle &H, const double *Qn, const double *Qin, double *Qout, int stage)
+{
+    switch (H.cfg.P) {
+    case 1: dg_stage_p<1>(H, Qn, Qin, Qout, stage); break;
+    case 2: dg_stage_p<2>(H, Qn, Qin, Qout, stage); break;
+    default: dg_stage_p<3>(H, Qn, Qin, Qout, stage); break;
+    }
+}
+
+void dg_max_speed(Handle &H, const double *Q)
+{
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, H.stream, H.lam_bits);
+    int blocks = (int)((H.g.np + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    switch (H.cfg.P) {
+    case 1:
+        hipLaunchKernelGGL((dg_max_speed_kernel<1>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+                           H.g.gamma, H.lam_bits);
+        break;
+    case 2:
+        hipLaunchKernelGGL((dg_max_speed_kernel<2>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+                           H.g.gamma, H.lam_bits);
+        break;
+    default:
+        hipLaunchKernelGGL((dg_max_speed_kernel<3>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+                           H.g.gamma, H.lam_bits);
+        break;
+    }
+}
+
+}  // namespace zk

@@ -1,0 +1,548 @@
+// zk_api.hip -- the C ABI of libzk.so (declared in include/zk.h).
+//
+// Host side of the drop-in boundary: owns device memory and one HIP stream
+// per handle, builds the ghost-fill source map once at load, and enqueues the
+// stage kernels.  The time loop stays on the device; the host only reads back
+// what the caller asks for.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "zk_internal.h"
+
+namespace zk {
+void dg_upload_tables(int device);
+}
+
+using zk::Handle;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+#define ZK_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (call);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            H->err = std::string(#call) + ": " + hipGetErrorString(_e);                   \
+            return ZK_ERR_HIP;                                                            \
+        }                                                                                 \
+    } while (0)
+
+__global__ void set_dt_kernel(zk::StepScalars *S, double dt)
+{
+    if (threadIdx.x == 0) {
+        S->dt = dt;
+        S->done = 0;
+    }
+}
+
+__global__ void advance_time_kernel(zk::StepScalars *S)
+{
+    if (threadIdx.x == 0 && !S->done) {
+        S->t += S->dt;
+        S->steps += 1;
+    }
+}
+
+// dt = cfl * da / lambda, clipped so that t + dt <= t_end; marks done once t
+// has reached t_end so that the stage kernels of later steps become no-ops.
+__global__ void cfl_dt_kernel(zk::StepScalars *S, const unsigned long long *lam_bits, double da)
+{
+    if (threadIdx.x != 0) return;
+    const double lam = __longlong_as_double((long long)*lam_bits);
+    S->lambda = lam;
+    if (S->t >= S->t_end) {
+        S->done = 1;
+        S->dt = 0.0;
+        return;
+    }
+    double dt = S->cfl * da / lam;
+    if (S->t + dt > S->t_end) dt = S->t_end - S->t;
+    S->dt = dt;
+}
+
+zk_status check_args(Handle *H)
+{
+    if (!H) return ZK_ERR_ARG;
+    if (!H->loaded) {
+        H->err = "state not loaded (call zk_load first)";
+        return ZK_ERR_ARG;
+    }
+    return ZK_OK;
+}
+
+void free_all(Handle *H)
+{
+    void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
+                    H->scal, H->lam_bits, H->bct};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    H->q = H->qa = H->qb = nullptr;
+    H->src = nullptr; H->upd = nullptr; H->ring = nullptr;
+    H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
+    H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
+    H->loaded = false;
+}
+
+// Ghost-fill source map: replay the row pass then the column pass of the
+// reference fill (src_WENO/fluid.cpp:338-370, src_DG/fluid.cpp:581-635) on
+// indices, so the last writer of every ghost is recorded; then periodic (-2)
+// ghosts take their periodic image inside [3, n-3).
+void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, bool slip, std::vector<int32_t> &src)
+{
+    src.resize((size_t)ny * nx);
+    for (size_t k = 0; k < src.size(); ++k) src[k] = (int32_t)(k << zk::kSrcShift);
+    const int OFF[6] = {-3, -2, -1, 1, 2, 3};
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 3; i < ny - 3; ++i)
+            for (int j = 3; j < nx - 3; ++j) {
+                const size_t k = (size_t)i * nx + j;
+                if (bct[k] != 0) continue;
+                for (int d = 0; d < 6; ++d) {
+                    int ni = i, nj = j;
+                    if (pass == 0) {
+                        ni = i + OFF[d];
+                        if ((unsigned)ni >= (unsigned)ny) continue;
+                    } else {
+                        nj = j + OFF[d];
+                        if ((unsigned)nj >= (unsigned)nx) continue;
+                    }
+                    const size_t dst = (size_t)ni * nx + nj;
+                    const int bt = bct[dst];
+                    if (bt == -1) src[dst] = (int32_t)(k << zk::kSrcShift);
+                    else if (bt == 1 && slip) src[dst] = (int32_t)((k << zk::kSrcShift) | (pass == 0 ? 2 : 1));
+                }
+            }
+    auto per = [](int i, int n) { return i < 3 ? i + (n - 6) : (i >= n - 3 ? i - (n - 6) : i); };
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            const size_t k = (size_t)i * nx + j;
+            if (bct[k] != -2) continue;
+            src[k] = (int32_t)(((size_t)per(i, ny) * nx + per(j, nx)) << zk::kSrcShift);
+        }
+}
+
+zk_status enqueue_step(Handle *H)
+{
+    if (H->cfg.method == ZK_METHOD_WENO) {
+        zk::fv_stage(*H, H->q, H->q, H->qa, 0);
+        zk::fv_stage(*H, H->q, H->qa, H->qb, 1);
+        zk::fv_stage(*H, H->q, H->qb, H->q, 2);
+    } else {
+        zk::dg_stage(*H, H->q, H->q, H->qa, 0);
+        zk::dg_stage(*H, H->q, H->qa, H->qb, 1);
+        zk::dg_stage(*H, H->q, H->qb, H->q, 2);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        H->err = std::string("kernel launch: ") + hipGetErrorString(e);
+        return ZK_ERR_HIP;
+    }
+    return ZK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *zk_last_error(const zk_handle *h)
+{
+    if (!h) return g_create_err.c_str();
+    return reinterpret_cast<const Handle *>(h)->err.c_str();
+}
+
+zk_status zk_create(const zk_config *cfg, zk_handle **out)
+{
+    g_create_err.clear();
+    if (!cfg || !out) {
+        g_create_err = "null argument";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->method != ZK_METHOD_WENO && cfg->method != ZK_METHOD_DG) {
+        g_create_err = "unknown method";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->method == ZK_METHOD_DG && (cfg->P < 1 || cfg->P > 3)) {
+        g_create_err = "DG degree P must be 1, 2 or 3";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->flux != ZK_FLUX_HLLC && cfg->flux != ZK_FLUX_HLL) {
+        g_create_err = "unknown flux";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->method == ZK_METHOD_DG && (cfg->limiter < 0 || cfg->limiter > ZK_LIMITER_LEARNED)) {
+        g_create_err = "unknown limiter";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->precision != ZK_PREC_FP64 && cfg->precision != ZK_PREC_FP32_BULK) {
+        g_create_err = "unknown precision";
+        return ZK_ERR_ARG;
+    }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+        return ZK_ERR_HIP;
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) {
+        g_create_err = "device ordinal out of range";
+        return ZK_ERR_ARG;
+    }
+    Handle *H = new Handle();
+    H->cfg = *cfg;
+    H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
+    e = hipSetDevice(cfg->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_err = std::string("stream creation: ") + hipGetErrorString(e);
+        delete H;
+        return ZK_ERR_HIP;
+    }
+    if (cfg->method == ZK_METHOD_DG) zk::dg_upload_tables(cfg->device);
+    *out = reinterpret_cast<zk_handle *>(H);
+    return ZK_OK;
+}
+
+zk_status zk_destroy(zk_handle *h)
+{
+    if (!h) return ZK_OK;
+    Handle *H = reinterpret_cast<Handle *>(h);
+    (void)hipSetDevice(H->cfg.device);
+    if (H->stream) (void)hipStreamSynchronize(H->stream);
+    free_all(H);
+    for (auto &r : H->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : H->pool) (void)hipEventDestroy(e);
+    if (H->stream) (void)hipStreamDestroy(H->stream);
+    delete H;
+    return ZK_OK;
+}
+
+zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const double *rho, const double *u,
+                  const double *v, const double *p, const int32_t *bctype)
+{
+    if (!h) return ZK_ERR_ARG;
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (ny < 7 || nx < 7 || !(da > 0) || !(gamma > 1.0) || !rho || !u || !v || !p || !bctype) {
+        H->err = "zk_load: need ny, nx >= 7, da > 0, gamma > 1 and non-null arrays";
+        return ZK_ERR_ARG;
+    }
+    if ((size_t)ny * nx >= ((size_t)1 << (31 - zk::kSrcShift))) {
+        H->err = "zk_load: grid too large for 32-bit source indices";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    if (H->stream) ZK_HIP(hipStreamSynchronize(H->stream));
+    free_all(H);
+    H->g.ny = ny; H->g.nx = nx; H->g.np = (size_t)ny * nx; H->g.da = da; H->g.gamma = gamma;
+    const size_t np = H->g.np, nst = 4 * (size_t)H->nm * np;
+
+    // conserved variables, src_WENO/fluid.cpp:86-93 (DG modes >= 1 are zero)
+    std::vector<double> Q(nst, 0.0);
+    for (size_t k = 0; k < np; ++k) {
+        const double r = rho[k], ux = u[k], vy = v[k], pr = p[k];
+        Q[k] = r;
+        Q[(size_t)H->nm * np + k] = r * ux;
+        Q[2 * (size_t)H->nm * np + k] = r * vy;
+        Q[3 * (size_t)H->nm * np + k] = pr / (gamma - 1.0) + 0.5 * r * (ux * ux + vy * vy);
+    }
+    H->h_bct.assign(bctype, bctype + np);
+    std::vector<int32_t> src;
+    build_src_map(H->h_bct, ny, nx, H->cfg.method == ZK_METHOD_DG, src);
+    std::vector<uint8_t> upd(np, 0);
+    std::vector<int32_t> ring;
+    int64_t nupd = 0;
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            const size_t k = (size_t)i * nx + j;
+            const bool in = i >= 3 && i < ny - 3 && j >= 3 && j < nx - 3;
+            if (in && bctype[k] == 0) {
+                upd[k] = 1;
+                ++nupd;
+            }
+            if (!in) ring.push_back((int32_t)k);
+        }
+    H->n_updated = nupd;
+    H->nring = (int)ring.size();
+
+    ZK_HIP(hipMalloc(&H->q, nst * sizeof(double)));
+    ZK_HIP(hipMalloc(&H->qa, nst * sizeof(double)));
+    ZK_HIP(hipMalloc(&H->qb, nst * sizeof(double)));
+    ZK_HIP(hipMalloc(&H->src, np * sizeof(int32_t)));
+    ZK_HIP(hipMalloc(&H->upd, np));
+    ZK_HIP(hipMalloc(&H->bct, np * sizeof(int32_t)));
+    ZK_HIP(hipMalloc(&H->ring, ring.size() * sizeof(int32_t)));
+    ZK_HIP(hipMalloc(&H->flags, np));
+    ZK_HIP(hipMalloc(&H->scal, sizeof(zk::StepScalars)));
+    ZK_HIP(hipMalloc(&H->lam_bits, sizeof(unsigned long long)));
+    if (H->cfg.method == ZK_METHOD_DG) {
+        const int NQ = H->cfg.P + 1;
+        const size_t nxu = nx - 6, nyu = ny - 6;
+        ZK_HIP(hipMalloc(&H->gx, 4 * (size_t)NQ * (nxu + 1) * nyu * sizeof(double)));
+        ZK_HIP(hipMalloc(&H->gy, 4 * (size_t)NQ * nxu * (nyu + 1) * sizeof(double)));
+        ZK_HIP(hipMalloc(&H->means, 4 * np * sizeof(double)));
+    }
+    ZK_HIP(hipMemcpy(H->q, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->qa, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->qb, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->src, src.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->upd, upd.data(), np, hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->bct, bctype, np * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!ring.empty()) ZK_HIP(hipMemcpy(H->ring, ring.data(), ring.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemset(H->flags, 0, np));
+    zk::StepScalars s0{};
+    ZK_HIP(hipMemcpy(H->scal, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemset(H->lam_bits, 0, sizeof(unsigned long long)));
+    ZK_HIP(hipDeviceSynchronize());
+    H->loaded = true;
+    return ZK_OK;
+}
+
+zk_status zk_set_state(zk_handle *h, const double *Q)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!Q) {
+        H->err = "zk_set_state: null array";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipMemcpyAsync(H->q, Q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyHostToDevice,
+                          H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return ZK_OK;
+}
+
+zk_status zk_get_state(zk_handle *h, double *Q)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!Q) {
+        H->err = "zk_get_state: null array";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipMemcpyAsync(Q, H->q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyDeviceToHost,
+                          H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return ZK_OK;
+}
+
+zk_status zk_get_conserved(zk_handle *h, double *U0, double *U1, double *U2, double *U3)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    double *dst[4] = {U0, U1, U2, U3};
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    for (int c = 0; c < 4; ++c) {
+        if (!dst[c]) {
+            H->err = "zk_get_conserved: null array";
+            return ZK_ERR_ARG;
+        }
+        ZK_HIP(hipMemcpyAsync(dst[c], H->q + (size_t)c * H->nm * H->g.np, H->g.np * sizeof(double),
+                              hipMemcpyDeviceToHost, H->stream));
+    }
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return ZK_OK;
+}
+
+zk_status zk_max_speed(zk_handle *h, double *out)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!out) {
+        H->err = "zk_max_speed: null output";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    if (H->cfg.method == ZK_METHOD_WENO) zk::fv_max_speed(*H, H->q);
+    else zk::dg_max_speed(*H, H->q);
+    ZK_HIP(hipGetLastError());
+    unsigned long long bits = 0;
+    ZK_HIP(hipMemcpyAsync(&bits, H->lam_bits, sizeof(bits), hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    double lam;
+    std::memcpy(&lam, &bits, sizeof(lam));
+    *out = lam;
+    return ZK_OK;
+}
+
+zk_status zk_step(zk_handle *h, double dt, int nsteps)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!(dt >= 0.0) || nsteps < 0) {
+        H->err = "zk_step: need dt >= 0 and nsteps >= 0";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, dt);
+    for (int s = 0; s < nsteps; ++s) {
+        st = enqueue_step(H);
+        if (st) return st;
+    }
+    return ZK_OK;
+}
+
+zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps, double *t_out, int *steps_out)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!(cfl > 0.0) || nsteps < 0) {
+        H->err = "zk_step_cfl: need cfl > 0 and nsteps >= 0";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    zk::StepScalars s{};
+    ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    s.cfl = cfl;
+    s.t_end = t_end;
+    s.done = 0;
+    const int steps0 = s.steps;
+    ZK_HIP(hipMemcpyAsync(H->scal, &s, sizeof(s), hipMemcpyHostToDevice, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    for (int k = 0; k < nsteps; ++k) {
+        if (H->cfg.method == ZK_METHOD_WENO) zk::fv_max_speed(*H, H->q);
+        else zk::dg_max_speed(*H, H->q);
+        hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, H->lam_bits, H->g.da);
+        // stage kernels read dt from device memory; a step after t_end has
+        // dt = 0 and done = 1, which the host detects below and stops.
+        if ((k & 15) == 15 || k == nsteps - 1) {
+            ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
+            ZK_HIP(hipStreamSynchronize(H->stream));
+            if (s.done) break;
+        }
+        st = enqueue_step(H);
+        if (st) return st;
+        hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, H->stream, H->scal);
+    }
+    ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    if (t_out) *t_out = s.t;
+    if (steps_out) *steps_out = s.steps - steps0;
+    return ZK_OK;
+}
+
+zk_status zk_sync(zk_handle *h)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return ZK_OK;
+}
+
+zk_status zk_step_timed(zk_handle *h, double dt, int nsteps, float *ms)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    hipEvent_t e0, e1;
+    ZK_HIP(hipEventCreate(&e0));
+    ZK_HIP(hipEventCreate(&e1));
+    ZK_HIP(hipEventRecord(e0, H->stream));
+    st = zk_step(h, dt, nsteps);
+    if (st) return st;
+    ZK_HIP(hipEventRecord(e1, H->stream));
+    ZK_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    ZK_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ZK_OK;
+}
+
+zk_status zk_get_flags(zk_handle *h, uint8_t *flags)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!flags) {
+        H->err = "zk_get_flags: null array";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipMemcpyAsync(flags, H->flags, H->g.np, hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return ZK_OK;
+}
+
+zk_status zk_profile(zk_handle *h, int enable)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    for (auto &r : H->recs) {
+        H->pool.push_back(r.a);
+        H->pool.push_back(r.b);
+    }
+    H->recs.clear();
+    H->prof = enable != 0;
+    return ZK_OK;
+}
+
+zk_status zk_profile_report(zk_handle *h, char *buf, size_t len)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H || !buf || len == 0) return ZK_ERR_ARG;
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<long> cnt;
+    for (auto &r : H->recs) {
+        float ms = 0.f;
+        ZK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        size_t i = 0;
+        while (i < names.size() && names[i] != r.name) ++i;
+        if (i == names.size()) {
+            names.emplace_back(r.name);
+            tot.push_back(0.0);
+            cnt.push_back(0);
+        }
+        tot[i] += ms;
+        cnt[i] += 1;
+    }
+    std::string js = "{";
+    for (size_t i = 0; i < names.size(); ++i) {
+        char tmp[256];
+        std::snprintf(tmp, sizeof(tmp), "%s\"%s\": [%ld, %.6f]", i ? ", " : "", names[i].c_str(), cnt[i], tot[i]);
+        js += tmp;
+    }
+    js += "}";
+    if (js.size() + 1 > len) {
+        H->err = "zk_profile_report: buffer too small";
+        return ZK_ERR_ARG;
+    }
+    std::memcpy(buf, js.c_str(), js.size() + 1);
+    return ZK_OK;
+}
+
+zk_status zk_updated_cells(zk_handle *h, int64_t *out)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (out) *out = H->n_updated;
+    return ZK_OK;
+}
+
+}  // extern "C"

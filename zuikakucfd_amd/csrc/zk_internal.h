@@ -1,0 +1,106 @@
+// zk_internal.h -- shared between the C-ABI layer and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/zk.h"
+
+namespace zk {
+
+// Ghost-fill map entry: (source cell index << 2) | negate bits.  Bit 0 negates
+// the x momentum, bit 1 the y momentum (slip ghosts).  A cell that is not
+// filled maps to itself.
+constexpr int kSrcShift = 2;
+
+// Per-step scalars that stage kernels read from device memory, so that a
+// captured graph replays with a dt the device itself computed.
+struct StepScalars {
+    double dt;        // time step of the current step
+    double t;         // physical time reached
+    double lambda;    // last max wave speed
+    double t_end;     // clip target for CFL stepping
+    double cfl;       // CFL number for CFL stepping
+    int32_t done;     // 1 once t >= t_end (CFL stepping stops updating)
+    int32_t steps;    // steps taken by CFL stepping
+};
+
+struct Grid {
+    int ny = 0, nx = 0;
+    size_t np = 0;        // ny*nx
+    double da = 0, gamma = 0;
+};
+
+struct Handle {
+    zk_config cfg{};
+    std::string err;
+    int nm = 1;           // modes per variable
+    Grid g;
+    hipStream_t stream = nullptr;
+    bool loaded = false;
+    int64_t n_updated = 0;
+
+    // device buffers
+    double *q = nullptr;      // state U^n (4*nm planes)
+    double *qa = nullptr;     // stage buffer A
+    double *qb = nullptr;     // stage buffer B
+    int32_t *src = nullptr;   // ghost-fill map (ny*nx)
+    uint8_t *upd = nullptr;   // 1 = updated cell
+    int32_t *ring = nullptr;  // indices of cells outside [3,ny-3)x[3,nx-3)
+    int nring = 0;
+    double *gx = nullptr;     // DG x-face projected fluxes
+    double *gy = nullptr;     // DG y-face projected fluxes
+    double *means = nullptr;  // DG stage means (4 planes) for the TVB limiter
+    uint8_t *flags = nullptr; // limiter flags
+    StepScalars *scal = nullptr;          // device
+    unsigned long long *lam_bits = nullptr; // device max-reduce target
+    int32_t *bct = nullptr;   // bctype copy (device)
+    std::vector<int32_t> h_bct;
+
+    // kernel timing: when prof is set every launcher brackets its kernel with
+    // a pair of events on the handle's stream
+    bool prof = false;
+    struct Rec { const char *name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+};
+
+// Brackets one kernel launch with events when profiling is enabled.
+struct ProfScope {
+    Handle &H;
+    int idx = -1;
+    ProfScope(Handle &h, const char *name) : H(h)
+    {
+        if (!H.prof) return;
+        hipEvent_t a, b;
+        if (H.pool.size() >= 2) {
+            a = H.pool.back(); H.pool.pop_back();
+            b = H.pool.back(); H.pool.pop_back();
+        } else {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+        }
+        H.recs.push_back({name, a, b});
+        idx = (int)H.recs.size() - 1;
+        (void)hipEventRecord(a, H.stream);
+    }
+    ~ProfScope()
+    {
+        if (idx >= 0) (void)hipEventRecord(H.recs[idx].b, H.stream);
+    }
+};
+
+// ---- launchers (defined in the kernel files) ----
+// WENO finite volume: one SSP-RK3 stage.  stage = 0, 1, 2.
+void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int stage);
+void fv_max_speed(Handle &H, const double *U);   // result in H.lam_bits
+void dg_stage(Handle &H, const double *Qn, const double *Qin, double *Qout, int stage);
+void dg_max_speed(Handle &H, const double *Q);   // result in H.lam_bits
+// CFL step control: dt = cfl*da/lambda (clip to t_end) computed on device.
+void cfl_dt(Handle &H, double cfl_limit_scale);
+void advance_time(Handle &H);
+
+}  // namespace zk

@@ -1,0 +1,244 @@
+"""Input/output formats of the reference solvers, and input generators.
+
+Inputs (src_WENO/fluid.cpp:33-94, src_DG/fluid.cpp:148-244):
+  * ``params.txt``: ``nx ny da gamma`` (nx first; fields are ny x nx);
+  * ``rho.dat u.dat v.dat p.dat``: whitespace-separated doubles read with
+    ``file >> mat(i, j)``, rows first;
+  * ``bctype.dat``: integers in the same order (0 updated, -1 copy ghost,
+    1 slip ghost for DG, -2 periodic ghost -- this project's extension).
+Outputs (saveMeshData, src_WENO/fluid.cpp:558-576): ``result/<step>/U{0..3}_<rank>.dat``,
+each the full ny x nx matrix printed with Eigen's default ``operator<<``:
+stream precision 6 (``%g``), every coefficient right-aligned to the widest
+one, single spaces between coefficients, rows separated by newlines and no
+trailing newline.
+
+The generators write inputs in the layout of the reference's generator
+notebook (gen.ipynb, cell 0): cell-centre values, row i=0 at the top
+(y = 1 - (i+0.5) da), a ring of three -1 ghost cells, ``%.10e`` values each
+followed by a space, one row per line.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+
+@dataclass
+class MeshData:
+    nx: int
+    ny: int
+    da: float
+    gamma: float
+    rho: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    p: np.ndarray
+    bctype: np.ndarray
+
+
+def _read_values(path: Path, n: int, dtype) -> np.ndarray:
+    if not path.exists():
+        raise FileNotFoundError(f"Cannot open file: {path}")
+    txt = path.read_text()
+    vals = np.array(txt.split()[:n], dtype=dtype)
+    if vals.size < n:
+        raise ValueError(f"{path}: expected {n} values, found {vals.size}")
+    return vals
+
+
+def read_mesh_folder(folder: str | os.PathLike) -> MeshData:
+    """Read params.txt and the five fields (Mesh(folderPath), src_WENO/fluid.cpp:68-94)."""
+    folder = Path(folder)
+    pf = folder / "params.txt"
+    if not pf.exists():
+        raise FileNotFoundError("Cannot open params.txt")
+    tok = pf.read_text().split()
+    nx, ny, da, gamma = int(tok[0]), int(tok[1]), float(tok[2]), float(tok[3])
+    n = nx * ny
+    f = {name: _read_values(folder / f"{name}.dat", n, np.float64).reshape(ny, nx) for name in ("rho", "u", "v", "p")}
+    bct = np.array(_read_values(folder / "bctype.dat", n, np.float64), dtype=np.int32).reshape(ny, nx)
+    return MeshData(nx, ny, da, gamma, f["rho"], f["u"], f["v"], f["p"], bct)
+
+
+def _fmt_g6(x: float) -> str:
+    return format(float(x), "g")
+
+
+def eigen_text(M: np.ndarray) -> str:
+    """Eigen 3 default ``operator<<`` text of a matrix (see module docstring)."""
+    M = np.asarray(M)
+    strs = [[_fmt_g6(x) for x in row] for row in M]
+    width = max((len(s) for row in strs for s in row), default=0)
+    return "\n".join(" ".join(s.rjust(width) for s in row) for row in strs)
+
+
+def save_mesh_data(U: np.ndarray, rank: int, folder: str | os.PathLike) -> None:
+    """Write U0..U3 as ``<folder>/U<c>_<rank>.dat`` (saveMeshData)."""
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    for c in range(4):
+        (folder / f"U{c}_{rank}.dat").write_text(eigen_text(U[c]))
+
+
+def save_binary_state(Q: np.ndarray, t: float, step: int, path: str | os.PathLike) -> None:
+    """Full-precision dump of every mode (no reference counterpart)."""
+    np.savez(path, Q=Q, t=t, step=step)
+
+
+# ---------------------------------------------------------------------------
+# writers in the generator-notebook layout
+# ---------------------------------------------------------------------------
+def _write_matrix(path: Path, data: np.ndarray, is_int: bool = False) -> None:
+    lines = []
+    for row in data:
+        if is_int:
+            lines.append("".join(f"{int(x)} " for x in row))
+        else:
+            lines.append("".join(f"{x:.10e} " for x in row))
+    path.write_text("\n".join(lines) + "\n")
+
+
+def write_mesh_folder(m: MeshData, folder: str | os.PathLike) -> None:
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    (folder / "params.txt").write_text(f"{m.nx} {m.ny} {m.da:.10e} {m.gamma}\n")
+    _write_matrix(folder / "rho.dat", m.rho)
+    _write_matrix(folder / "u.dat", m.u)
+    _write_matrix(folder / "v.dat", m.v)
+    _write_matrix(folder / "p.dat", m.p)
+    _write_matrix(folder / "bctype.dat", m.bctype, is_int=True)
+
+
+def ring_bctype(ny: int, nx: int, code: int = -1) -> np.ndarray:
+    b = np.zeros((ny, nx), dtype=np.int32)
+    b[:3, :] = code
+    b[-3:, :] = code
+    b[:, :3] = code
+    b[:, -3:] = code
+    return b
+
+
+# ---------------------------------------------------------------------------
+# generators
+# ---------------------------------------------------------------------------
+STD_QUADRANTS = {
+    # (rho, u, v_phys, p) per quadrant; v in the files is the velocity along +i
+    # (downward), so v_file = -v_phys
+    "UR": (1.5, 0.0, 0.0, 1.5),
+    "UL": (0.5323, 1.206, 0.0, 0.3),
+    "LL": (0.138, 1.206, 1.206, 0.029),
+    "LR": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def riemann_2d(n: int = 256, split=(0.8, 0.8), gamma: float = 1.4, quadrants: dict | None = None) -> MeshData:
+    """Four-quadrant Riemann problem on [0,1]^2 (gen.ipynb cell 0 layout).
+
+    Quadrant states are (rho, u, v_phys, p); a cell belongs to the upper/right
+    side when its centre is >= the split coordinate, as in the notebook.
+    """
+    q = dict(STD_QUADRANTS if quadrants is None else quadrants)
+    da = 1.0 / n
+    j = np.arange(n)
+    i = np.arange(n)
+    xc = (j + 0.5) * da
+    yc = 1.0 - (i + 0.5) * da
+    XC, YC = np.meshgrid(xc, yc)
+    right = XC >= split[0]
+    up = YC >= split[1]
+    rho = np.zeros((n, n)); u = np.zeros((n, n)); v = np.zeros((n, n)); p = np.zeros((n, n))
+    for name, mask in (("UR", right & up), ("UL", ~right & up), ("LL", ~right & ~up), ("LR", right & ~up)):
+        r_, u_, vphys, p_ = q[name]
+        rho[mask] = r_
+        u[mask] = u_
+        v[mask] = -vphys if vphys != 0.0 else 0.0
+        p[mask] = p_
+    return MeshData(n, n, da, gamma, rho, u, v, p, ring_bctype(n, n))
+
+
+def random_quadrants(seed: int) -> tuple[dict, tuple[float, float]]:
+    """Seeded random four-quadrant states: rho ~ U[0.1,2], p ~ U[0.05,2],
+    u, v ~ U[-1.5,1.5] per quadrant, split ~ U[0.3,0.7]^2."""
+    rng = np.random.default_rng(seed)
+    q = {}
+    for name in ("UR", "UL", "LL", "LR"):
+        rho = rng.uniform(0.1, 2.0)
+        p = rng.uniform(0.05, 2.0)
+        u = rng.uniform(-1.5, 1.5)
+        v = rng.uniform(-1.5, 1.5)
+        q[name] = (rho, u, v, p)
+    split = (rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7))
+    return q, split
+
+
+def random_riemann(seed: int, n: int, gamma: float = 1.4) -> MeshData:
+    q, split = random_quadrants(seed)
+    return riemann_2d(n, split=split, gamma=gamma, quadrants=q)
+
+
+def jet(nx: int = 2048, ny: int = 1024, gamma: float = 5.0 / 3.0, amb=(0.5, 0.0, 0.0, 0.4127),
+        inflow=(5.0, 30.0, 0.0, 0.4127), half_width: float = 0.05) -> MeshData:
+    """High-Mach jet on [0,2]x[-0.5,0.5]: ambient state, and the inflow rows
+    (|y_c| < half_width) of columns 0-2 held fixed as bctype-0 cells outside
+    the update range -- the mechanism of the reference's HighMachJet input."""
+    da = 1.0 / ny
+    rho = np.full((ny, nx), amb[0]); u = np.full((ny, nx), amb[1])
+    v = np.full((ny, nx), amb[2]); p = np.full((ny, nx), amb[3])
+    bct = ring_bctype(ny, nx)
+    yc = 0.5 - (np.arange(ny) + 0.5) * da
+    rows = np.nonzero(np.abs(yc) < half_width)[0]
+    rho[rows, :3] = inflow[0]; u[rows, :3] = inflow[1]; v[rows, :3] = inflow[2]; p[rows, :3] = inflow[3]
+    bct[rows, :3] = 0
+    return MeshData(nx, ny, da, gamma, rho, u, v, p, bct)
+
+
+def smooth_wave(n: int = 512, amp: float = 0.2, gamma: float = 1.4) -> MeshData:
+    """rho = 1 + amp sin(2 pi (x + y)), u = v = 1, p = 1 on the periodic unit
+    square (solver coordinates: x along j, y along +i), ring of -2 ghosts,
+    h = 1/(n-6) so the updated cells span exactly one period.  Cell values are
+    point values at the centres; use `smooth_wave_modes` for DG projections."""
+    h = 1.0 / (n - 6)
+    c = (np.arange(n) - 3 + 0.5) * h
+    X, Y = np.meshgrid(c, c)
+    rho = 1.0 + amp * np.sin(2 * np.pi * (X + Y))
+    one = np.ones((n, n))
+    return MeshData(n, n, h, gamma, rho, one.copy(), one.copy(), one.copy(), ring_bctype(n, n, -2))
+
+
+def smooth_wave_modes(n: int, P: int, amp: float = 0.2, gamma: float = 1.4, t: float = 0.0) -> np.ndarray:
+    """L2 projection of the smooth wave (translated by (t, t)) onto the
+    tensor Legendre modes m = px*(P+1)+py, shape (4, nm, n, n).  Every
+    conserved variable is affine in rho (rho u = rho v = rho, E = 1/(g-1) + rho).
+    Uses 8-point Gauss quadrature per direction, exact to round-off here."""
+    h = 1.0 / (n - 6)
+    nq = 8
+    xg, wg = np.polynomial.legendre.leggauss(nq)
+    c = (np.arange(n) - 3 + 0.5) * h
+    NQ = P + 1
+    Lx = np.array([np.polynomial.legendre.Legendre.basis(k)(xg) for k in range(NQ)])  # (NQ, nq)
+    # rho at quadrature points of every cell: x = c_j + xi h/2, y = c_i + eta h/2
+    xs = c[None, :, None] + 0.5 * h * xg[None, None, :]   # (1, n, nq) along j
+    ys = c[:, None, None] + 0.5 * h * xg[None, None, :]   # (n, 1, nq) along i
+    # rho(x,y) = 1 + amp sin(2pi(x+y-2t)) = 1 + amp [sin(A)cos(B) + cos(A)sin(B)]
+    A = 2 * np.pi * (xs - t)  # (1, n, nq)
+    B = 2 * np.pi * (ys - t)  # (n, 1, nq)
+    sA = np.einsum("xjq,pq->xjp", np.sin(A) , Lx * wg) ; cA = np.einsum("xjq,pq->xjp", np.cos(A), Lx * wg)
+    sB = np.einsum("iyq,pq->iyp", np.sin(B), Lx * wg) ; cB = np.einsum("iyq,pq->iyp", np.cos(B), Lx * wg)
+    nm = NQ * NQ
+    R = np.zeros((nm, n, n))
+    for px in range(NQ):
+        for py in range(NQ):
+            norm = (2 * px + 1) * (2 * py + 1) / 4.0
+            proj = sA[0, :, px][None, :] * cB[:, 0, py][:, None] + cA[0, :, px][None, :] * sB[:, 0, py][:, None]
+            R[px * NQ + py] = norm * amp * proj
+    R[0] += 1.0
+    Q = np.zeros((4, nm, n, n))
+    Q[0] = R
+    Q[1] = R
+    Q[2] = R
+    Q[3] = R
+    Q[3, 0] += 1.0 / (gamma - 1.0)
+    return Q

@@ -1,0 +1,209 @@
+"""ctypes binding of libzk.so (the C ABI declared in include/zk.h).
+
+This is the Python side of the drop-in boundary: `Solver` owns one zk handle
+(one whole-domain state on one GPU).  Arrays cross the boundary as host numpy
+buffers that the library copies; nothing here computes the update itself.
+If libzk.so is missing or no GPU is visible the constructor raises -- there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libzk.so"
+
+METHOD_WENO, METHOD_DG = 0, 1
+PREC_FP64, PREC_FP32_BULK = 0, 1
+FLUX_HLLC, FLUX_HLL = 0, 1
+LIMITER_NONE, LIMITER_PERSSON_ZS, LIMITER_TVB, LIMITER_LEARNED = 0, 1, 2, 3
+
+ZK_OK = 0
+_ERR_NAMES = {1: "ZK_ERR_ARG", 2: "ZK_ERR_HIP", 3: "ZK_ERR_NUMERIC", 4: "ZK_ERR_CFL"}
+
+EXPORTED = [
+    "zk_create", "zk_destroy", "zk_last_error", "zk_load", "zk_set_state", "zk_get_state",
+    "zk_get_conserved", "zk_max_speed", "zk_step", "zk_step_cfl", "zk_sync", "zk_step_timed",
+    "zk_get_flags", "zk_updated_cells", "zk_profile", "zk_profile_report",
+]
+
+
+class ZkConfig(C.Structure):
+    _fields_ = [
+        ("method", C.c_int),
+        ("P", C.c_int),
+        ("precision", C.c_int),
+        ("flux", C.c_int),
+        ("limiter", C.c_int),
+        ("tvb_M", C.c_double),
+        ("device", C.c_int),
+    ]
+
+
+class ZkError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libzk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ZkError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(str(p))
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    lib.zk_create.argtypes = [C.POINTER(ZkConfig), C.POINTER(vp)]
+    lib.zk_destroy.argtypes = [vp]
+    lib.zk_last_error.argtypes = [vp]
+    lib.zk_last_error.restype = C.c_char_p
+    lib.zk_load.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, ip]
+    lib.zk_set_state.argtypes = [vp, dp]
+    lib.zk_get_state.argtypes = [vp, dp]
+    lib.zk_get_conserved.argtypes = [vp, dp, dp, dp, dp]
+    lib.zk_max_speed.argtypes = [vp, dp]
+    lib.zk_step.argtypes = [vp, C.c_double, C.c_int]
+    lib.zk_step_cfl.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, C.POINTER(C.c_int)]
+    lib.zk_sync.argtypes = [vp]
+    lib.zk_step_timed.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_float)]
+    lib.zk_get_flags.argtypes = [vp, C.POINTER(C.c_uint8)]
+    lib.zk_updated_cells.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.zk_profile.argtypes = [vp, C.c_int]
+    lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
+    for name in EXPORTED:
+        if name != "zk_last_error":
+            getattr(lib, name).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One whole-domain explicit Euler state on one GPU.
+
+    Mirrors the reference's per-rank `Mesh` + `updateMesh` pair
+    (src_WENO/fluid.h:26-80,218; src_DG/fluid.h:44-110,160) behind the C ABI.
+    """
+
+    def __init__(self, method: int = METHOD_WENO, P: int = 1, precision: int = PREC_FP64,
+                 flux: int = FLUX_HLLC, limiter: int = LIMITER_PERSSON_ZS, tvb_M: float = 50.0,
+                 device: int = 0):
+        self.lib = load_library()
+        self.cfg = ZkConfig(method, P, precision, flux, limiter, float(tvb_M), device)
+        h = C.c_void_p()
+        st = self.lib.zk_create(C.byref(self.cfg), C.byref(h))
+        if st != ZK_OK:
+            raise ZkError(f"zk_create: {_ERR_NAMES.get(st, st)}: {self.lib.zk_last_error(None).decode()}")
+        self.h = h
+        self.method = method
+        self.P = P
+        self.nm = (P + 1) ** 2 if method == METHOD_DG else 1
+        self.ny = self.nx = 0
+
+    # -- lifetime -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.zk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != ZK_OK:
+            msg = self.lib.zk_last_error(self.h).decode()
+            raise ZkError(f"{what}: {_ERR_NAMES.get(st, st)}: {msg}")
+
+    # -- data -----------------------------------------------------------
+    def load(self, da: float, gamma: float, rho, u, v, p, bctype) -> None:
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        ny, nx = rho.shape
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(ny, nx) for a in (u, v, p)]
+        bc = np.ascontiguousarray(bctype, dtype=np.int32).reshape(ny, nx)
+        self._check(self.lib.zk_load(self.h, ny, nx, float(da), float(gamma), _dptr(rho), *map(_dptr, arrs),
+                                     bc.ctypes.data_as(C.POINTER(C.c_int32))), "zk_load")
+        self.ny, self.nx = ny, nx
+        self.da, self.gamma = float(da), float(gamma)
+
+    def set_state(self, Q: np.ndarray) -> None:
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if Q.size != 4 * self.nm * self.ny * self.nx:
+            raise ValueError("state has the wrong size")
+        self._check(self.lib.zk_set_state(self.h, _dptr(Q)), "zk_set_state")
+
+    def get_state(self) -> np.ndarray:
+        Q = np.empty((4, self.nm, self.ny, self.nx), dtype=np.float64)
+        self._check(self.lib.zk_get_state(self.h, _dptr(Q)), "zk_get_state")
+        return Q
+
+    def get_conserved(self) -> np.ndarray:
+        U = np.empty((4, self.ny, self.nx), dtype=np.float64)
+        self._check(self.lib.zk_get_conserved(self.h, *(_dptr(U[c]) for c in range(4))), "zk_get_conserved")
+        return U
+
+    def get_flags(self) -> np.ndarray:
+        f = np.zeros((self.ny, self.nx), dtype=np.uint8)
+        self._check(self.lib.zk_get_flags(self.h, f.ctypes.data_as(C.POINTER(C.c_uint8))), "zk_get_flags")
+        return f
+
+    def updated_cells(self) -> int:
+        n = C.c_int64()
+        self._check(self.lib.zk_updated_cells(self.h, C.byref(n)), "zk_updated_cells")
+        return n.value
+
+    # -- compute --------------------------------------------------------
+    def max_speed(self) -> float:
+        out = C.c_double()
+        self._check(self.lib.zk_max_speed(self.h, C.byref(out)), "zk_max_speed")
+        return out.value
+
+    def step(self, dt: float, nsteps: int = 1) -> None:
+        self._check(self.lib.zk_step(self.h, float(dt), int(nsteps)), "zk_step")
+
+    def step_cfl(self, cfl: float, t_end: float, nsteps: int) -> tuple[float, int]:
+        t = C.c_double()
+        n = C.c_int()
+        self._check(self.lib.zk_step_cfl(self.h, float(cfl), float(t_end), int(nsteps), C.byref(t), C.byref(n)),
+                    "zk_step_cfl")
+        return t.value, n.value
+
+    def step_timed(self, dt: float, nsteps: int) -> float:
+        ms = C.c_float()
+        self._check(self.lib.zk_step_timed(self.h, float(dt), int(nsteps), C.byref(ms)), "zk_step_timed")
+        return ms.value
+
+    def profile(self, enable: bool = True) -> None:
+        self._check(self.lib.zk_profile(self.h, 1 if enable else 0), "zk_profile")
+
+    def profile_report(self) -> dict:
+        import json
+
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.lib.zk_profile_report(self.h, buf, len(buf)), "zk_profile_report")
+        return {k: {"launches": v[0], "total_ms": v[1]} for k, v in json.loads(buf.value.decode()).items()}
+
+    def sync(self) -> None:
+        self._check(self.lib.zk_sync(self.h), "zk_sync")
