@@ -30,6 +30,7 @@ def lib() -> C.CDLL:
         L.zko_weno5.argtypes = [dp, C.POINTER(d), C.POINTER(d)]
         for n in ("zko_hllc_fv", "zko_hllc_dg", "zko_hll"):
             getattr(L, n).argtypes = [dp, dp, d, dp]
+        L.zko_hllc_speeds.argtypes = [i, dp, dp, d, dp]
         L.zko_fv_fill_ghosts.argtypes = [i, i, ip, dp]
         L.zko_fv_rhs.argtypes = [i, i, d, d, ip, dp, d, i, dp]
         L.zko_fv_step.argtypes = [i, i, d, d, ip, dp, d, i]
@@ -67,6 +68,13 @@ def flux(kind: str, L, R, gamma: float) -> np.ndarray:
     getattr(lib(), fn)(np.ascontiguousarray(L, dtype=np.float64), np.ascontiguousarray(R, dtype=np.float64),
                        float(gamma), F)
     return F
+
+
+def hllc_speeds(kind: int, L, R, gamma: float) -> np.ndarray:
+    out = np.zeros(3)
+    lib().zko_hllc_speeds(int(kind), np.ascontiguousarray(L, dtype=np.float64),
+                          np.ascontiguousarray(R, dtype=np.float64), float(gamma), out)
+    return out
 
 
 def prim_to_cons(gamma, rho, u, v, p) -> np.ndarray:

@@ -119,6 +119,35 @@ void zko_hllc_fv(const double L[4], const double R[4], double gamma, double F[4]
     }
 }
 
+/* Wave-speed estimates (SL, SR, S*) of the two HLLC variants, exposed so the
+ * tests can check them against closed forms.  kind 0: finite volume
+ * (src_WENO/fluid.cpp:274-300), kind 1: DG (src_DG/fluid.cpp:456-496). */
+void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamma, double out[3])
+{
+    const double eps = kind == 0 ? 1e-12 : 1e-5;
+    double rhoL = stdmax(L[0], eps), uL = L[1] / rhoL, vL = L[2] / rhoL, EL = L[3] / rhoL;
+    double pL = stdmax((gamma - 1.0) * (L[3] - 0.5 * rhoL * (uL * uL + vL * vL)), eps);
+    double rhoR = stdmax(R[0], eps), uR = R[1] / rhoR, vR = R[2] / rhoR, ER = R[3] / rhoR;
+    double pR = stdmax((gamma - 1.0) * (R[3] - 0.5 * rhoR * (uR * uR + vR * vR)), eps);
+    double aL, aR;
+    if (kind == 0) { aL = sqrt(gamma * pL / rhoL); aR = sqrt(gamma * pR / rhoR); }
+    else { aL = sqrt(stdmax(gamma * pL / rhoL, eps)); aR = sqrt(stdmax(gamma * pR / rhoR, eps)); }
+    double sqL = sqrt(rhoL), sqR = sqrt(rhoR), den = sqL + sqR;
+    if (kind == 1 && den < eps) den = eps;
+    double u_roe = (sqL * uL + sqR * uR) / den;
+    double h_roe = (sqL * (EL + pL / rhoL) + sqR * (ER + pR / rhoR)) / den;
+    double c_roe = sqrt(stdmax((gamma - 1.0) * (h_roe - 0.5 * u_roe * u_roe), eps));
+    double SL = stdmin(uL - aL, u_roe - c_roe);
+    double SR = stdmax(uR + aR, u_roe + c_roe);
+    double num = pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR);
+    double den2 = rhoL * (SL - uL) - rhoR * (SR - uR);
+    double g = kind == 0 ? 1e-12 : eps;
+    if (fabs(den2) < g) den2 = (den2 >= 0 ? g : -g);
+    out[0] = SL;
+    out[1] = SR;
+    out[2] = num / den2;
+}
+
 /* HLLC flux of the DG solver: eps 1e-5 floors, sound-speed floor, Roe
  * denominator guard, S - S* guard and NaN/inf guard on the star term.
  * Follows src_DG/fluid.cpp:450-520. */

@@ -36,6 +36,7 @@ void zko_weno5(const double s[6], double *UL, double *UR);
 void zko_hllc_fv(const double L[4], const double R[4], double gamma, double F[4]);
 void zko_hllc_dg(const double L[4], const double R[4], double gamma, double F[4]);
 void zko_hll(const double L[4], const double R[4], double gamma, double F[4]);
+void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamma, double out[3]);
 
 /* ---------------- finite volume (WENO5) ---------------- */
 /* flux: 0 = HLLC (src_WENO/fluid.cpp:268-317), 1 = HLL (src_DG/fluid.cpp:525-580) */

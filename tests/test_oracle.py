@@ -1,0 +1,462 @@
+"""Pins of the CPU oracle (oracle/zk_oracle.c) by closed forms, invariants and
+brute force.  The reference ships no tests and cannot be compiled here (it
+needs Eigen3), so these checks are what fixes the restatement; each is chosen
+so that a plausible slip (a dropped term, a wrong sign, index or constant)
+fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import updated_mask
+from zuikakucfd_amd import meshio
+
+G = 1.4
+
+
+def cons(rho, u, v, p, g=G):
+    return np.array([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def phys_flux(U, g=G):
+    rho, mu, mv, E = U
+    u, v = mu / rho, mv / rho
+    p = (g - 1) * (E - 0.5 * rho * (u * u + v * v))
+    return np.array([rho * u, rho * u * u + p, rho * u * v, (E + p) * u])
+
+
+# ---------------------------------------------------------------- WENO5
+def cell_avg_poly(coef, k):
+    """Average over [k, k+1] of sum_n coef[n] x^n."""
+    return sum(c * ((k + 1) ** (n + 1) - k ** (n + 1)) / (n + 1) for n, c in enumerate(coef))
+
+
+@pytest.mark.parametrize("coef", [(1.0,), (0.3, -2.0), (0.5, 1.5, -0.75), (-2.0, 0.1, 3.0)])
+def test_weno5_exact_for_quadratics(coef):
+    """Every candidate stencil reproduces cell averages of a quadratic exactly,
+    so both face values equal the polynomial at the face whatever the weights."""
+    s = [cell_avg_poly(coef, k) for k in range(6)]  # cells [0,1] .. [5,6]; face at x = 3
+    exact = sum(c * 3.0 ** n for n, c in enumerate(coef))
+    UL, UR = oracle.weno5(s)
+    assert UL == pytest.approx(exact, rel=1e-12, abs=1e-12)
+    assert UR == pytest.approx(exact, rel=1e-12, abs=1e-12)
+
+
+def test_weno5_mirror_symmetry():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        s = rng.normal(size=6)
+        UL, UR = oracle.weno5(s)
+        UL2, UR2 = oracle.weno5(s[::-1])
+        assert UR == pytest.approx(UL2, rel=1e-13, abs=1e-15)
+        assert UL == pytest.approx(UR2, rel=1e-13, abs=1e-15)
+
+
+def test_weno5_essentially_non_oscillatory_at_a_jump():
+    UL, UR = oracle.weno5([0, 0, 0, 1, 1, 1])
+    assert abs(UL) < 1e-9 and abs(UR - 1) < 1e-9
+    UL, UR = oracle.weno5([2, 2, 2, 2, -1, -1])  # jump one cell to the right of the face
+    assert abs(UL - 2) < 1e-9
+
+
+def test_weno5_fifth_order():
+    """Face values of cell averages of sin(x) converge at 5th order (ideal
+    weights d = 0.1, 0.6, 0.3; a wrong weight or stencil drops to 3rd)."""
+    errs = []
+    for h in (0.2, 0.1, 0.05):
+        x0 = 0.3
+        s = [(np.cos(x0 + k * h) - np.cos(x0 + (k + 1) * h)) / h for k in range(6)]
+        UL, UR = oracle.weno5(s)
+        exact = np.sin(x0 + 3 * h)
+        errs.append(max(abs(UL - exact), abs(UR - exact)))
+    r1, r2 = np.log2(errs[0] / errs[1]), np.log2(errs[1] / errs[2])
+    assert r1 > 4.5 and r2 > 4.5, (errs, r1, r2)
+
+
+# ---------------------------------------------------------------- fluxes
+STATES = [(1.0, 0.3, -0.2, 1.0), (0.138, 1.206, -1.206, 0.029), (5.0, 30.0, 0.0, 0.4127), (0.5, -0.7, 2.0, 0.3)]
+
+
+@pytest.mark.parametrize("kind", ["hllc_fv", "hllc_dg", "hll"])
+@pytest.mark.parametrize("st", STATES)
+def test_flux_consistency(kind, st):
+    g = 5 / 3 if st[0] == 5.0 else G
+    U = cons(*st, g=g)
+    F = oracle.flux(kind, U, U, g)
+    np.testing.assert_allclose(F, phys_flux(U, g), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["hllc_fv", "hllc_dg", "hll"])
+def test_flux_supersonic_upwind(kind):
+    L = cons(1.0, 5.0, 0.1, 1.0)
+    R = cons(0.5, 4.0, -0.2, 0.4)
+    np.testing.assert_array_equal(oracle.flux(kind, L, R, G), phys_flux(L))
+    L2 = cons(1.0, -5.0, 0.1, 1.0)
+    R2 = cons(0.5, -4.0, -0.2, 0.4)
+    np.testing.assert_allclose(oracle.flux(kind, L2, R2, G), phys_flux(R2), rtol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["hllc_fv", "hllc_dg"])
+def test_hllc_resolves_stationary_contact(kind):
+    """A stationary contact (u = 0, equal p, different rho) gives F = (0, p, 0, 0)."""
+    L = cons(1.0, 0.0, 0.3, 0.7)
+    R = cons(0.2, 0.0, -0.5, 0.7)
+    F = oracle.flux(kind, L, R, G)
+    np.testing.assert_allclose(F, [0.0, 0.7, 0.0, 0.0], atol=1e-14)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_hllc_roe_sound_speed_ignores_v(kind):
+    """For equal states the Roe averages are the state itself; the reference
+    forms c_roe from h_roe - u_roe^2/2 only (src_WENO/fluid.cpp:289), so
+    c_roe^2 = a^2 + (g-1) v^2 / 2 and SL = u - c_roe, SR = u + c_roe, S* = u."""
+    rho, u, v, p = 0.8, 0.4, 1.3, 0.9
+    U = cons(rho, u, v, p)
+    SL, SR, Ss = oracle.hllc_speeds(kind, U, U, G)
+    c = np.sqrt(G * p / rho + (G - 1) * v * v / 2)
+    assert SL == pytest.approx(u - c, rel=1e-13)
+    assert SR == pytest.approx(u + c, rel=1e-13)
+    assert Ss == pytest.approx(u, rel=1e-12)
+
+
+def test_hll_rotational_symmetry():
+    """HLL uses both Roe velocity components: swapping the tangential sign
+    leaves the normal flux unchanged and flips the tangential momentum flux."""
+    L = cons(1.0, 0.2, 0.5, 1.0)
+    R = cons(0.4, -0.1, 0.9, 0.3)
+    F = oracle.flux("hll", L, R, G)
+    Lm = L.copy(); Lm[2] *= -1
+    Rm = R.copy(); Rm[2] *= -1
+    Fm = oracle.flux("hll", Lm, Rm, G)
+    np.testing.assert_allclose(Fm[[0, 1, 3]], F[[0, 1, 3]], rtol=1e-14)
+    assert Fm[2] == pytest.approx(-F[2], rel=1e-14)
+
+
+# ---------------------------------------------------------------- ghost fill
+def fill_closed_form(bct):
+    """Source of every filled ghost by the characterisation of the reference
+    fill (row pass then column pass, last writer wins): in the row pass ghost
+    (g, j) takes the LARGEST i in [3, ny-4] with bctype(i, j) == 0 and
+    1 <= |i - g| <= 3 (j in [3, nx-4]); the column pass overrides with the
+    largest valid j.  Returns {ghost: source}."""
+    ny, nx = bct.shape
+    src = {}
+    for g in range(ny):
+        for j in range(3, nx - 3):
+            if bct[g, j] != -1:
+                continue
+            cands = [i for i in range(3, ny - 3) if bct[i, j] == 0 and 1 <= abs(i - g) <= 3]
+            if cands:
+                src[(g, j)] = (max(cands), j)
+    for i in range(3, ny - 3):
+        for g in range(nx):
+            if bct[i, g] != -1:
+                continue
+            cands = [j for j in range(3, nx - 3) if bct[i, j] == 0 and 1 <= abs(j - g) <= 3]
+            if cands:
+                src[(i, g)] = (i, max(cands))
+    return src
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fv_ghost_fill_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    ny, nx = 14, 17
+    bct = meshio.ring_bctype(ny, nx)
+    if seed:
+        holes = rng.random((ny, nx)) < 0.15
+        bct[holes] = -1
+    U = np.stack([np.arange(ny * nx, dtype=np.float64).reshape(ny, nx) + 1000 * c for c in range(4)])
+    o = oracle.FV(1.0, G, U, bct)
+    o.fill()
+    expect = U.copy()
+    for (g, j), (i, jj) in fill_closed_form(bct).items():
+        expect[:, g, j] = U[:, i, jj]
+    np.testing.assert_array_equal(o.U, expect)
+
+
+def test_standard_ring_fill_is_shift_then_constant():
+    """Standard 3-ring: ghost rows 0,1,2 <- rows 3,4,5; ghost rows ny-3.. <- row ny-4;
+    same for columns; corners untouched."""
+    ny, nx = 12, 13
+    U = np.stack([np.arange(ny * nx, dtype=np.float64).reshape(ny, nx) for _ in range(4)])
+    o = oracle.FV(1.0, G, U, meshio.ring_bctype(ny, nx))
+    o.fill()
+    A = o.U[0]
+    for g in range(3):
+        np.testing.assert_array_equal(A[g, 3:nx - 3], U[0, g + 3, 3:nx - 3])
+        np.testing.assert_array_equal(A[ny - 1 - g, 3:nx - 3], U[0, ny - 4, 3:nx - 3])
+        np.testing.assert_array_equal(A[3:ny - 3, g], U[0, 3:ny - 3, g + 3])
+        np.testing.assert_array_equal(A[3:ny - 3, nx - 1 - g], U[0, 3:ny - 3, nx - 4])
+    np.testing.assert_array_equal(A[:3, :3], U[0, :3, :3])
+
+
+def test_dg_slip_fill_negates_normal_momentum():
+    ny, nx, P = 10, 11, 1
+    bct = meshio.ring_bctype(ny, nx)
+    bct[:3, :] = 1          # top rows: slip, y normal
+    bct[3:ny - 3, -3:] = 1  # right columns: slip, x normal
+    rng = np.random.default_rng(3)
+    Q = rng.normal(size=(4, 4, ny, nx))
+    o = oracle.DG(P, 1.0, G, Q, bct)
+    o.fill()
+    # top ghost row 1 <- row 4 with rho*v negated
+    np.testing.assert_array_equal(o.Q[[0, 1, 3], :, 1, 4], Q[[0, 1, 3], :, 4, 4])
+    np.testing.assert_array_equal(o.Q[2, :, 1, 4], -Q[2, :, 4, 4])
+    # right ghost column nx-2 <- column nx-4 with rho*u negated
+    np.testing.assert_array_equal(o.Q[1, :, 5, nx - 2], -Q[1, :, 5, nx - 4])
+    np.testing.assert_array_equal(o.Q[2, :, 5, nx - 2], Q[2, :, 5, nx - 4])
+
+
+def test_periodic_fill():
+    n = 12
+    bct = meshio.ring_bctype(n, n, -2)
+    U = np.stack([np.arange(n * n, dtype=np.float64).reshape(n, n)] * 4)
+    o = oracle.FV(1.0, G, U, bct)
+    o.fill()
+    A = o.U[0]
+    np.testing.assert_array_equal(A[0:3, 3:n - 3], U[0, n - 6:n - 3, 3:n - 3])
+    np.testing.assert_array_equal(A[n - 3:, 3:n - 3], U[0, 3:6, 3:n - 3])
+    np.testing.assert_array_equal(A[3:n - 3, 0:3], U[0, 3:n - 3, n - 6:n - 3])
+
+
+# ---------------------------------------------------------------- time integration
+def test_stale_ghost_mixture():
+    """With a constant interior C and ghosts G != C, one step leaves every
+    filled ghost at 1/3 G + 2/3 C (U^n is copied before the fill; dU = 0 on
+    ghosts; stage combinations 3/4, 1/4 and 1/3, 2/3)."""
+    n = 12
+    st = cons(1.0, 0.2, -0.1, 1.0)
+    U = np.broadcast_to(st[:, None, None], (4, n, n)).copy()
+    bct = meshio.ring_bctype(n, n)
+    Gv = cons(0.5, 0.0, 0.0, 0.3)
+    ring = bct != 0
+    U[:, ring] = Gv[:, None]
+    o = oracle.FV(1.0 / n, G, U, bct)
+    o.step(1e-3)
+    filled = np.zeros((n, n), dtype=bool)
+    filled[:3, 3:n - 3] = filled[-3:, 3:n - 3] = filled[3:n - 3, :3] = filled[3:n - 3, -3:] = True
+    expect = (1.0 / 3.0) * Gv + (2.0 / 3.0) * st
+    np.testing.assert_allclose(o.U[:, filled], np.broadcast_to(expect[:, None], (4, filled.sum())), rtol=1e-15)
+    # interior unchanged (free stream)
+    np.testing.assert_allclose(o.U[:, 3:n - 3, 3:n - 3], U[:, 3:n - 3, 3:n - 3], rtol=1e-14)
+    # corners are never filled and keep a convex mix of themselves
+    np.testing.assert_allclose(o.U[:, 0, 0], Gv, rtol=1e-15)
+
+
+def smooth_fv(n, t=0.0):
+    """Exact cell averages of rho = 1 + 0.2 sin(2 pi (x + y - 2t)), u = v = 1, p = 1."""
+    h = 1.0 / (n - 6)
+    c = (np.arange(n) - 3 + 0.5) * h
+    X, Y = np.meshgrid(c, c)
+    fac = (np.sin(np.pi * h) / (np.pi * h)) ** 2
+    rho = 1.0 + 0.2 * fac * np.sin(2 * np.pi * (X + Y - 2 * t))
+    U = np.stack([rho, rho, rho, 1.0 / (G - 1) + rho])
+    return U, h
+
+
+def test_fv_spatial_order_smooth_periodic():
+    """WENO5 + HLLC on the periodic smooth wave: density error falls at
+    (at least) 4th order between 20^2 and 40^2 updated cells."""
+    errs = []
+    for n in (26, 46):
+        U, h = smooth_fv(n)
+        bct = meshio.ring_bctype(n, n, -2)
+        o = oracle.FV(h, G, U, bct)
+        T = 0.05
+        nsteps = int(np.ceil(T / (0.05 * h)))
+        o.step(T / nsteps, nsteps)
+        Ue, _ = smooth_fv(n, T)
+        errs.append(np.abs(o.U[0, 3:n - 3, 3:n - 3] - Ue[0, 3:n - 3, 3:n - 3]).mean())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 4.0, (errs, rate)
+
+
+def test_rk3_temporal_order():
+    """SSP-RK3: with the spatial operator fixed, the error against a dt/16
+    solution falls 8x per halving of dt (a wrong stage weight gives 2x or 4x)."""
+    n = 22
+    U0, h = smooth_fv(n)
+    bct = meshio.ring_bctype(n, n, -2)
+    T = 0.2
+
+    def run(k):
+        o = oracle.FV(h, G, U0, bct)
+        o.step(T / k, k)
+        return o.U[0, 3:n - 3, 3:n - 3]  # ghosts hold stale stage mixtures, not the solution
+
+    ref = run(640)
+    e = [np.abs(run(k) - ref).max() for k in (20, 40)]
+    ratio = e[0] / e[1]
+    assert 6.5 < ratio < 9.5, (e, ratio)
+
+
+def test_fv_max_speed_includes_ghosts():
+    n = 10
+    U = np.broadcast_to(cons(1.0, 0.0, 0.0, 1.0)[:, None, None], (4, n, n)).copy()
+    U[:, 0, 0] = cons(1.0, 3.0, 4.0, 1.0)  # corner ghost, never updated
+    o = oracle.FV(0.1, G, U, meshio.ring_bctype(n, n))
+    assert o.max_speed() == pytest.approx(5.0 + np.sqrt(G), rel=1e-15)
+
+
+# ---------------------------------------------------------------- DG
+def test_legendre_and_gauss_tables():
+    L = oracle.lib()
+    for nq in range(1, 6):
+        pts = np.ctypeslib.as_array(L.zko_gl_points(nq), shape=(nq,))
+        wts = np.ctypeslib.as_array(L.zko_gl_weights(nq), shape=(nq,))
+        x, w = np.polynomial.legendre.leggauss(nq)
+        np.testing.assert_allclose(pts, x, atol=2e-16)
+        np.testing.assert_allclose(wts, w, atol=2e-16)
+    xs = np.linspace(-1, 1, 7)
+    for n in range(5):
+        Pn = np.polynomial.legendre.Legendre.basis(n)
+        for x in xs:
+            assert L.zko_legP(n, x) == pytest.approx(Pn(x), abs=1e-15)
+            assert L.zko_legD(n, x) == pytest.approx(Pn.deriv()(x), abs=1e-14)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_dg_cfl_limit(P):
+    assert oracle.dg_cfl_limit(P) == pytest.approx(0.9 / (2 * (2 * P + 1)), rel=1e-15)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_dg_free_stream(P):
+    n = 10
+    nm = (P + 1) ** 2
+    Q = np.zeros((4, nm, n, n))
+    Q[:, 0] = cons(0.7, 0.3, -0.4, 1.1)[:, None, None]
+    o = oracle.DG(P, 0.1, G, Q, meshio.ring_bctype(n, n))
+    dQ = o.rhs()
+    assert np.abs(dQ).max() < 1e-13
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_dg_conservation_periodic(P):
+    """With periodic ghosts the face terms cancel pairwise: the sum of mode 0
+    over the updated cells is conserved by a step (limiter off)."""
+    n = 14
+    m = meshio.smooth_wave(n)
+    Q0 = meshio.smooth_wave_modes(n, P)
+    rng = np.random.default_rng(P)
+    Q0[:, 1:] += 0.01 * rng.normal(size=Q0[:, 1:].shape)
+    o = oracle.DG(P, m.da, G, Q0, m.bctype, limiter=0)
+    s0 = Q0[:, 0, 3:n - 3, 3:n - 3].sum(axis=(1, 2))
+    o.step(0.02 * m.da)
+    s1 = o.Q[:, 0, 3:n - 3, 3:n - 3].sum(axis=(1, 2))
+    np.testing.assert_allclose(s1, s0, rtol=1e-13)
+
+
+@pytest.mark.parametrize("P,ns,min_rate", [(1, (14, 22), 1.8), (2, (12, 18), 2.7), (3, (10, 14), 3.5)])
+def test_dg_order_smooth_periodic(P, ns, min_rate):
+    """Order P+1 on the smooth periodic wave with projected initial modes and
+    the limiter off (a sign slip in a face or volume term destroys it)."""
+    errs = []
+    hs = []
+    for n in ns:
+        m = meshio.smooth_wave(n)
+        Q0 = meshio.smooth_wave_modes(n, P)
+        o = oracle.DG(P, m.da, G, Q0, m.bctype, limiter=0)
+        T = 0.05
+        lam = o.max_speed()
+        k = int(np.ceil(T / (0.5 * oracle.dg_cfl_limit(P) * m.da / lam)))
+        o.step(T / k, k)
+        Qe = meshio.smooth_wave_modes(n, P, t=T)
+        errs.append(np.abs(o.Q[0, 0, 3:n - 3, 3:n - 3] - Qe[0, 0, 3:n - 3, 3:n - 3]).mean())
+        hs.append(m.da)
+    rate = np.log(errs[0] / errs[1]) / np.log(hs[0] / hs[1])
+    assert rate > min_rate, (errs, rate)
+
+
+def test_tvb_limiter_closed_forms():
+    """Linear data with |slope coefficient| <= M h^2 is never limited; above it
+    minmod keeps the coefficient while both neighbour differences exceed it,
+    and a jump is limited to the smaller difference with higher modes zeroed."""
+    P, n = 2, 12
+    nm = 9
+    h = 0.1
+    bct = meshio.ring_bctype(n, n)
+    j = np.arange(n)
+    # smooth linear ramp in x: means 0.01*j, x-linear coefficient 0.005 (= slope h/2)
+    Q = np.zeros((4, nm, n, n))
+    Q[:, 0] = 1.0 + 0.01 * j[None, None, :]
+    Q[:, P + 1] = 0.005
+    Q[:, 4] = 1e-4  # a higher mode
+    o = oracle.DG(P, h, G, Q, bct, tvb_M=50.0)
+    f = o.limit_tvb()
+    assert f.sum() == 0
+    np.testing.assert_array_equal(o.Q, Q)
+    # M = 0: |a| <= 0 fails, minmod(0.005, 0.01, 0.01) = 0.005 unchanged -> no flag
+    # inside; next to the ghost ring the filled means (left: column j+3, a
+    # shift; right: column nx-4, constant) break monotonicity, so the first and
+    # last updated columns are limited
+    o2 = oracle.DG(P, h, G, Q, bct, tvb_M=0.0)
+    f2 = o2.limit_tvb()
+    assert f2[3:n - 3, 4:n - 4].sum() == 0
+    assert f2[3:n - 3, 3].all() and f2[3:n - 3, n - 4].all()
+    # a jump between columns 5 and 6 with an overshooting slope in cell (6, 5)
+    Q3 = np.zeros((4, nm, n, n))
+    Q3[:, 0] = np.where(j[None, None, :] >= 6, 2.0, 1.0)
+    Q3[:, P + 1, 6, 5] = 0.3
+    Q3[:, 4, 6, 5] = 0.05
+    o3 = oracle.DG(P, h, G, Q3, bct, tvb_M=0.0)
+    f3 = o3.limit_tvb()
+    assert f3[6, 5] == 1 and f3.sum() == 1
+    # minmod(0.3, 1.0, 0.0) = 0: the slope is removed and every higher mode zeroed
+    assert np.all(o3.Q[:, 1:, 6, 5] == 0.0)
+
+
+def test_persson_closed_forms():
+    P, n = 2, 8
+    nm = 9
+    bct = meshio.ring_bctype(n, n)
+    Q = np.zeros((4, nm, n, n))
+    Q[:, 0] = 1.0
+    Q[0, 8, 4, 4] = 1.5     # mode (2,2): Ep/Etot = 2.25/3.25 -> s > s0 + kappa -> eps = 1
+    Q[0, 1, 5, 5] = 0.3     # only a linear mode: Ep = 0 -> untouched
+    Q[3, 0] = 2.5           # keep pressure positive
+    o = oracle.DG(P, 0.1, G, Q, bct)
+    o.limit_persson_zs()
+    assert o.Q[0, 8, 4, 4] == 0.0
+    assert o.Q[0, 1, 5, 5] == 0.3
+    # intermediate energy ratio: s = log10(Ep/(Etot+1e-5)) between s0 and s0+1
+    s0 = -4 * np.log10(2)
+    target = s0 + 0.5
+    a2 = 10 ** target * (1.0 + 1e-5) / (1 - 10 ** target)  # Ep/(1+Ep+1e-5) = 10^target
+    Q2 = np.zeros((4, nm, n, n)); Q2[:, 0] = 1.0; Q2[3, 0] = 2.5
+    Q2[0, 2, 4, 4] = np.sqrt(a2)  # mode (0,2) is in the px==P or py==P set
+    o2 = oracle.DG(P, 0.1, G, Q2, bct)
+    o2.limit_persson_zs()
+    eps = 0.5 * (1 + np.sin(np.pi * (0.5 - 0.5)))  # r = 0.5
+    assert o2.Q[0, 2, 4, 4] == pytest.approx(np.sqrt(a2) * (1 - eps), rel=1e-12)
+
+
+def test_zhang_shu_density_scaling():
+    """A negative density at a Gauss point scales modes >= 1 by
+    theta = (rho0 - 1e-5) / (rho0 - rho_min + 1e-5)."""
+    P, n = 1, 8
+    bct = meshio.ring_bctype(n, n)
+    Q = np.zeros((4, 4, n, n))
+    Q[0, 0] = 1.0; Q[3, 0] = 2.5
+    Q[0, 2, 4, 4] = 1.5   # x-linear mode: rho(xi) = 1 + 1.5 xi, min at xi = -0.8611...
+    o = oracle.DG(P, 0.1, G, Q, bct)
+    o.limit_zs()
+    rmin = 1.0 - 1.5 * 0.8611363115940526
+    theta = (1.0 - 1e-5) / (1.0 - rmin + 1e-5)
+    assert o.Q[0, 2, 4, 4] == pytest.approx(1.5 * theta, rel=1e-13)
+
+
+def test_dg_max_speed_corners():
+    P, n = 1, 8
+    Q = np.zeros((4, 4, n, n))
+    Q[:, 0] = cons(1.0, 0.5, 0.0, 1.0)[:, None, None]
+    bct = meshio.ring_bctype(n, n)
+    o = oracle.DG(P, 0.1, G, Q, bct)
+    assert o.max_speed() == pytest.approx(0.5 + np.sqrt(G), rel=1e-15)
+    # an x-linear velocity mode raises the corner speed at xi = +1
+    Q[1, 2, 4, 4] = 0.2
+    o = oracle.DG(P, 0.1, G, Q, bct)
+    u = 0.7
+    p = (G - 1) * (Q[3, 0, 4, 4] - 0.5 * 0.7 ** 2)
+    assert o.max_speed() == pytest.approx(u + np.sqrt(G * p), rel=1e-14)

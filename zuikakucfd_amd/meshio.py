@@ -91,25 +91,27 @@ def save_binary_state(Q: np.ndarray, t: float, step: int, path: str | os.PathLik
 # ---------------------------------------------------------------------------
 # writers in the generator-notebook layout
 # ---------------------------------------------------------------------------
-def _write_matrix(path: Path, data: np.ndarray, is_int: bool = False) -> None:
-    lines = []
-    for row in data:
-        if is_int:
-            lines.append("".join(f"{int(x)} " for x in row))
-        else:
-            lines.append("".join(f"{x:.10e} " for x in row))
+def _write_matrix(path: Path, data: np.ndarray, is_int: bool = False, joined: bool = False) -> None:
+    """Row-per-line text; each value followed by a space (the notebook's
+    writer) or, with joined, single spaces between values only (the layout of
+    the reference's HighMachJet input)."""
+    fmt = (lambda x: f"{int(x)}") if is_int else (lambda x: f"{x:.10e}")
+    if joined:
+        lines = [" ".join(fmt(x) for x in row) for row in data]
+    else:
+        lines = ["".join(fmt(x) + " " for x in row) for row in data]
     path.write_text("\n".join(lines) + "\n")
 
 
-def write_mesh_folder(m: MeshData, folder: str | os.PathLike) -> None:
+def write_mesh_folder(m: MeshData, folder: str | os.PathLike, joined: bool = False) -> None:
     folder = Path(folder)
     folder.mkdir(parents=True, exist_ok=True)
     (folder / "params.txt").write_text(f"{m.nx} {m.ny} {m.da:.10e} {m.gamma}\n")
-    _write_matrix(folder / "rho.dat", m.rho)
-    _write_matrix(folder / "u.dat", m.u)
-    _write_matrix(folder / "v.dat", m.v)
-    _write_matrix(folder / "p.dat", m.p)
-    _write_matrix(folder / "bctype.dat", m.bctype, is_int=True)
+    _write_matrix(folder / "rho.dat", m.rho, joined=joined)
+    _write_matrix(folder / "u.dat", m.u, joined=joined)
+    _write_matrix(folder / "v.dat", m.v, joined=joined)
+    _write_matrix(folder / "p.dat", m.p, joined=joined)
+    _write_matrix(folder / "bctype.dat", m.bctype, is_int=True, joined=joined)
 
 
 def ring_bctype(ny: int, nx: int, code: int = -1) -> np.ndarray:
@@ -179,19 +181,26 @@ def random_riemann(seed: int, n: int, gamma: float = 1.4) -> MeshData:
     return riemann_2d(n, split=split, gamma=gamma, quadrants=q)
 
 
-def jet(nx: int = 2048, ny: int = 1024, gamma: float = 5.0 / 3.0, amb=(0.5, 0.0, 0.0, 0.4127),
-        inflow=(5.0, 30.0, 0.0, 0.4127), half_width: float = 0.05) -> MeshData:
-    """High-Mach jet on [0,2]x[-0.5,0.5]: ambient state, and the inflow rows
-    (|y_c| < half_width) of columns 0-2 held fixed as bctype-0 cells outside
-    the update range -- the mechanism of the reference's HighMachJet input."""
-    da = 1.0 / ny
+def jet(nx: int = 2048, ny: int = 1024, da: float | None = None, gamma: float = 5.0 / 3.0,
+        amb=(0.5, 0.0, 0.0, 0.4127), inflow=(5.0, 30.0, 0.0, 0.4127), half_width: float = 0.05,
+        fixed_left: bool = False) -> MeshData:
+    """High-Mach jet: ambient state; in columns 0-2 the rows with
+    |y_c| < half_width (y_c = (ny/2 - (i+0.5)) da) carry the inflow state as
+    bctype-0 cells outside the update range, which are never updated -- the
+    fixed-Dirichlet mechanism of the reference's HighMachJet input.  With
+    fixed_left the whole left strip (rows 3..ny-4) is fixed as in that input;
+    otherwise the rest of the left edge is a -1 (transmissive) ghost."""
+    da = 1.0 / ny if da is None else da
     rho = np.full((ny, nx), amb[0]); u = np.full((ny, nx), amb[1])
     v = np.full((ny, nx), amb[2]); p = np.full((ny, nx), amb[3])
     bct = ring_bctype(ny, nx)
-    yc = 0.5 - (np.arange(ny) + 0.5) * da
+    yc = (ny / 2 - (np.arange(ny) + 0.5)) * da
     rows = np.nonzero(np.abs(yc) < half_width)[0]
     rho[rows, :3] = inflow[0]; u[rows, :3] = inflow[1]; v[rows, :3] = inflow[2]; p[rows, :3] = inflow[3]
-    bct[rows, :3] = 0
+    if fixed_left:
+        bct[3:ny - 3, :3] = 0
+    else:
+        bct[rows, :3] = 0
     return MeshData(nx, ny, da, gamma, rho, u, v, p, bct)
 
 
