@@ -60,6 +60,9 @@ typedef struct zk_config {
     int limiter;     /* ZK_LIMITER_*; DG only */
     double tvb_M;    /* TVB constant M (north_star: 50) */
     int device;      /* HIP device ordinal */
+    int positivity;  /* WENO: 1 = face positivity check (FP64): a face whose reconstructed
+                        states have rho or p below 1e-12 falls back to first order.
+                        No reference counterpart; 0 reproduces the reference scheme. */
 } zk_config;
 
 /* Create a handle.  Replaces the `Mesh` object's lifetime

@@ -85,8 +85,12 @@ def prim_to_cons(gamma, rho, u, v, p) -> np.ndarray:
     return U
 
 
+POSITIVITY = 16
+
+
 class FV:
-    """Finite-volume oracle state: U has shape (4, ny, nx)."""
+    """Finite-volume oracle state: U has shape (4, ny, nx).  flux may be or-ed
+    with POSITIVITY to apply the face positivity fallback."""
 
     def __init__(self, da, gamma, U, bctype, flux: int = 0):
         self.U = np.ascontiguousarray(U, dtype=np.float64).copy()

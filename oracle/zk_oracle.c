@@ -329,11 +329,32 @@ void zko_dg_fill_ghosts(int ny, int nx, int P, const int32_t *bct, double *Q)
 
 /* dU = dt * L(U) on bctype-0 cells in [3,ny-3)x[3,nx-3), 0 elsewhere; the
  * ghosts of U are filled first.  src_WENO/fluid.cpp:328-463. */
+/* Face-level positivity check (an option of this project, no reference
+ * counterpart): when either reconstructed face state has rho or p below
+ * 1e-12, the face falls back to the first-order states (the two cell values). */
+static inline int face_positive(const double Q[4], double gamma)
+{
+    if (!(Q[0] >= 1e-12)) return 0;
+    double p = (gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+    return p >= 1e-12;
+}
+
+static inline void positivity_fix(double QL[4], double QR[4], const double cl[4], const double cr[4], double gamma)
+{
+    if (face_positive(QL, gamma) && face_positive(QR, gamma)) return;
+    for (int c = 0; c < 4; ++c) {
+        QL[c] = cl[c];
+        QR[c] = cr[c];
+    }
+}
+
 void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
                 double *U, double dt, int flux, double *dU)
 {
     const size_t NP = (size_t)ny * nx;
     const double dx = da, dy = da;
+    const int pos = (flux & ZKO_POSITIVITY) != 0;
+    flux &= ~ZKO_POSITIVITY;
     flux_fn fl = flux == 1 ? zko_hll : zko_hllc_fv;
     zko_fv_fill_ghosts(ny, nx, bct, U);
     memset(dU, 0, 4 * NP * sizeof(double));
@@ -361,6 +382,12 @@ void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
                 zko_weno5(s, &QLd[c], &QRd[c]);          /* face i+1/2 */
                 for (int t = 0; t < 6; ++t) s[t] = Y[t][c];
                 zko_weno5(s, &QLu[c], &QRu[c]);          /* face i-1/2 */
+            }
+            if (pos) {
+                positivity_fix(QLr, QRr, X[3], X[4], gamma);
+                positivity_fix(QLl, QRl, X[2], X[3], gamma);
+                positivity_fix(QLd, QRd, Y[3], Y[4], gamma);
+                positivity_fix(QLu, QRu, Y[2], Y[3], gamma);
             }
             double Fr[4], Fl[4], Fd[4], Fu[4];
             fl(QLr, QRr, gamma, Fr);

@@ -39,7 +39,9 @@ void zko_hll(const double L[4], const double R[4], double gamma, double F[4]);
 void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamma, double out[3]);
 
 /* ---------------- finite volume (WENO5) ---------------- */
-/* flux: 0 = HLLC (src_WENO/fluid.cpp:268-317), 1 = HLL (src_DG/fluid.cpp:525-580) */
+/* flux: 0 = HLLC (src_WENO/fluid.cpp:268-317), 1 = HLL (src_DG/fluid.cpp:525-580);
+ * or-ed with ZKO_POSITIVITY it also applies the face positivity fallback. */
+#define ZKO_POSITIVITY 16
 void zko_fv_fill_ghosts(int ny, int nx, const int32_t *bct, double *U /*4 planes*/);
 void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
                 double *U, double dt, int flux, double *dU);

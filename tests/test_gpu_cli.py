@@ -1,0 +1,70 @@
+"""Drop-in executables: same command line, inputs and result/<step>/U*_0.dat
+outputs as solver_WENO / solver_DG4 for one rank, checked against the oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import rel_l1
+from zuikakucfd_amd import meshio
+
+pytestmark = pytest.mark.gpu
+BIN = Path(__file__).resolve().parents[1] / "zuikakucfd_amd" / "bin"
+
+
+def run(args, cwd):
+    print(f"[case] {' '.join(map(str, args))}", flush=True)
+    return subprocess.run([str(a) for a in args], cwd=cwd, capture_output=True, text=True, timeout=120)
+
+
+def read_dump(path):
+    raw = Path(path).read_bytes()
+    ny, nx, nm, _ = np.frombuffer(raw[:16], dtype=np.int32)
+    return np.frombuffer(raw[16:], dtype=np.float64).reshape(4, nm, ny, nx)
+
+
+def test_zk_weno_fixed_dt(tmp_path):
+    m = meshio.riemann_2d(64)
+    meshio.write_mesh_folder(m, tmp_path / "mesh")
+    r = run([BIN / "zk_weno", "mesh", "2e-4", "150", "--dump", "--quiet"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    # outputs at step 100 and at the last step, as src_WENO/solver.cpp:193
+    assert (tmp_path / "result" / "100" / "U0_0.dat").exists()
+    got = read_dump(tmp_path / "result" / "150" / "state.f64")[:, 0]
+    mm = meshio.read_mesh_folder(tmp_path / "mesh")
+    o = oracle.FV(mm.da, mm.gamma, oracle.prim_to_cons(mm.gamma, mm.rho, mm.u, mm.v, mm.p), mm.bctype)
+    o.step(2e-4, 150)
+    for c in range(4):
+        assert rel_l1(got[c], o.U[c]) <= 1e-10
+        txt = np.array((tmp_path / "result" / "150" / f"U{c}_0.dat").read_text().split(), dtype=float)
+        np.testing.assert_allclose(txt.reshape(64, 64), o.U[c], rtol=1e-5, atol=1e-12)
+
+
+def test_zk_dg_cfl_abort(tmp_path):
+    meshio.write_mesh_folder(meshio.riemann_2d(32), tmp_path / "mesh")
+    r = run([BIN / "zk_dg4", "mesh", "0.01", "5"], tmp_path)
+    assert r.returncode == 1 and "CFL" in r.stderr
+
+
+def test_zk_dg_auto(tmp_path):
+    m = meshio.riemann_2d(40)
+    meshio.write_mesh_folder(m, tmp_path / "mesh")
+    r = run([BIN / "zk_dg3", "mesh", "auto", "25", "--dump", "--quiet"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = read_dump(tmp_path / "result" / "25" / "state.f64")
+    mm = meshio.read_mesh_folder(tmp_path / "mesh")
+    o = oracle.DG.from_prims(2, mm.da, mm.gamma, mm.rho, mm.u, mm.v, mm.p, mm.bctype, limiter=1)
+    # auto mode: dt from the CFL limit, rechecked every 10 steps (src_DG/solver.cpp:270-289)
+    for step in range(25):
+        if step % 10 == 0:
+            dt = max(oracle.dg_cfl_limit(2) * mm.da / max(o.max_speed(), 1e-7), 1e-6)
+        o.step(dt)
+    for c in range(4):
+        assert rel_l1(got[c], o.Q[c]) <= 1e-10
+
+
+def test_missing_input_fails(tmp_path):
+    r = run([BIN / "zk_weno", "nowhere", "1e-4", "10"], tmp_path)
+    assert r.returncode != 0 and "Cannot open" in r.stderr

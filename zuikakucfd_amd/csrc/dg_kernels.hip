@@ -31,6 +31,8 @@ constexpr int DG_MAXNM = 16;
 struct DgTables {
     double w[DG_MAXNQ];                              // GL weights
     double wl[DG_MAXNQ][DG_MAXNQ];                   // w_q * L_p(s_q)  [p][q]
+    double wdl[DG_MAXNQ][DG_MAXNQ];                  // w_q * L'_p(s_q) [p][q]
+    double lq[DG_MAXNQ][DG_MAXNQ];                   // L_p(s_q)        [p][q]
     double sgn[DG_MAXNQ];                            // L_p(-1) = (-1)^p
     double tr_x[2][DG_MAXNQ][DG_MAXNM];              // phi_m(-1 / +1, eta_q)
     double tr_y[2][DG_MAXNQ][DG_MAXNM];              // phi_m(xi_q, -1 / +1)
@@ -93,7 +95,11 @@ DgTables make_tables(int P)
     const double zsp[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
     for (int q = 0; q < NQ; ++q) {
         T.w[q] = gw[q];
-        for (int p = 0; p < NQ; ++p) T.wl[p][q] = gw[q] * legP_h(p, gp[q]);
+        for (int p = 0; p < NQ; ++p) {
+            T.wl[p][q] = gw[q] * legP_h(p, gp[q]);
+            T.wdl[p][q] = gw[q] * legD_h(p, gp[q]);
+            T.lq[p][q] = legP_h(p, gp[q]);
+        }
     }
     for (int p = 0; p < NQ; ++p) T.sgn[p] = legP_h(p, -1.0);
     const double XI[4] = {-1.0, +1.0, -1.0, +1.0}, ETA[4] = {-1.0, -1.0, +1.0, +1.0};
@@ -161,6 +167,7 @@ struct DgArgs {
     const int32_t *bct;
     const StepScalars *S;
     double *gx, *gy;
+    double *means;    // stage means (4 planes) for the TVB limiter, or null
     int ny, nx, nxu, nyu;
     size_t np, nfx, nfy;
     double da, gamma;
@@ -168,12 +175,42 @@ struct DgArgs {
     int stage, flux, limiter;
 };
 
+template <int P, bool X>
+__device__ __forceinline__ void load_trace(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
+                                           size_t k, bool plus, double C[4][P + 1])
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const int32_t e = src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) C[c][t] = 0.0;
+#pragma unroll
+        for (int px = 0; px < NQ; ++px)
+#pragma unroll
+            for (int py = 0; py < NQ; ++py) {
+                const double a = Q[(size_t)(c * NM + px * NQ + py) * np + s];
+                const int t = X ? py : px;          // direction along the face
+                const int o = X ? px : py;          // direction across the face
+                if (plus || (o & 1) == 0) C[c][t] += a;
+                else C[c][t] -= a;
+            }
+    }
+    if (e & 1)
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) C[1][t] = -C[1][t];
+    if (e & 2)
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) C[2][t] = -C[2][t];
+}
+
 // One thread per face.  X: faces between (i, j-1) and (i, j), j in [3, nx-3];
 // otherwise faces between (i-1, j) and (i, j), i in [3, ny-3].
 template <int P, bool X>
 __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
 {
-    constexpr int NQ = P + 1, NM = NQ * NQ;
+    constexpr int NQ = P + 1;
     const DgTables &T = c_tab[P - 1];
     const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nf = X ? A.nfx : A.nfy;
@@ -189,9 +226,12 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
         i = 3 + (int)(f / w); j = 3 + (int)(f % w);
         kl = (size_t)(i - 1) * A.nx + j; kr = (size_t)i * A.nx + j;
     }
-    double AL[4][NM], AR[4][NM];
-    load_cell<P>(A.Qin, A.src, A.np, kl, AL);
-    load_cell<P>(A.Qin, A.src, A.np, kr, AR);
+    // Trace coefficients along the face.  On an x face the left cell's trace
+    // at xi = +1 is sum_py (sum_px a[px][py]) L_py(eta) and the right cell's
+    // at xi = -1 weights a[px][py] by (-1)^px; on a y face px and py swap.
+    double CL[4][NQ], CR[4][NQ];
+    load_trace<P, X>(A.Qin, A.src, A.np, kl, true, CL);
+    load_trace<P, X>(A.Qin, A.src, A.np, kr, false, CR);
 
     double g[4][NQ];
 #pragma unroll
@@ -203,15 +243,12 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
     for (int q = 0; q < NQ; ++q) {
         double UL[4] = {0.0, 0.0, 0.0, 0.0}, UR[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            const double bl = X ? T.tr_x[1][q][m] : T.tr_y[1][q][m];
-            const double br = X ? T.tr_x[0][q][m] : T.tr_y[0][q][m];
+        for (int t = 0; t < NQ; ++t)
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                UL[c] += AL[c][m] * bl;
-                UR[c] += AR[c][m] * br;
+                UL[c] += CL[c][t] * T.lq[t][q];
+                UR[c] += CR[c][t] * T.lq[t][q];
             }
-        }
         double Fs[4];
         if (X) {
             if (A.flux == ZK_FLUX_HLL) hll_f64(UL, UR, A.gamma, Fs);
@@ -237,11 +274,13 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
 
 // Positivity scaling of one cell (src_DG/fluid.cpp:897-991).
 template <int P>
-__device__ __forceinline__ void zs_cell(double Q[4][(P + 1) * (P + 1)], double gamma)
+__device__ __forceinline__ bool zs_cell(double Q[4][(P + 1) * (P + 1)], double gamma)
 {
+    bool changed = false;
     constexpr int NM = (P + 1) * (P + 1);
     const DgTables &T = c_tab[P - 1];
     constexpr double RHO_MIN = 1e-5, P_MIN = 1e-5, EPS = 1e-5;
+    if (Q[0][0] < RHO_MIN) changed = true;
     Q[0][0] = smax(Q[0][0], RHO_MIN);
     const double rho0 = Q[0][0], mx0 = Q[1][0], my0 = Q[2][0];
     double E0 = Q[3][0];
@@ -249,6 +288,7 @@ __device__ __forceinline__ void zs_cell(double Q[4][(P + 1) * (P + 1)], double g
     if (p0 < P_MIN) {
         E0 = P_MIN / (gamma - 1.0) + 0.5 * (mx0 * mx0 + my0 * my0) / rho0;
         Q[3][0] = E0;
+        changed = true;
     }
     double theta_rho = 1.0;
     for (int a = 0; a < 4; a++)
@@ -261,11 +301,13 @@ __device__ __forceinline__ void zs_cell(double Q[4][(P + 1) * (P + 1)], double g
                 theta_rho = smin(theta_rho, t);
             }
         }
-    if (theta_rho < 1.0)
+    if (theta_rho < 1.0) {
+        changed = true;
 #pragma unroll
         for (int m = 1; m < NM; m++)
 #pragma unroll
             for (int c = 0; c < 4; ++c) Q[c][m] *= theta_rho;
+    }
 
     double theta_p = 1.0;
     for (int a = 0; a < 4; a++)
@@ -288,11 +330,14 @@ __device__ __forceinline__ void zs_cell(double Q[4][(P + 1) * (P + 1)], double g
                 theta_p = smin(theta_p, t);
             }
         }
-    if (theta_p < 1.0)
+    if (theta_p < 1.0) {
+        changed = true;
 #pragma unroll
         for (int m = 1; m < NM; m++)
 #pragma unroll
             for (int c = 0; c < 4; ++c) Q[c][m] *= theta_p;
+    }
+    return changed;
 }
 
 // Persson modal-decay filter of one cell (src_DG/fluid.cpp:841-887).
@@ -353,26 +398,55 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
         for (int m = 0; m < NM; ++m) rhs[c][m] = 0.0;
 
     if (upd) {
-        // volume term
+        // volume term, sum-factorised: for each xi_a collapse px, evaluate the
+        // NQ points of that column, project along eta, then along xi.
+        //   rhs[c][px,py] += sum_a w_a L'_px(xi_a) SF_a[c][py] + w_a L_px(xi_a) SG_a[c][py]
+        //   SF_a[c][py] = sum_b w_b L_py(eta_b) F_c(a,b),  SG_a = sum_b w_b L'_py(eta_b) G_c(a,b)
 #pragma unroll
-        for (int a = 0; a < NQ; ++a)
+        for (int a = 0; a < NQ; ++a) {
+            double Y[4][NQ];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int py = 0; py < NQ; ++py) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int px = 0; px < NQ; ++px) acc += Q[c][px * NQ + py] * T.lq[px][a];
+                    Y[c][py] = acc;
+                }
+            double SF[4][NQ], SG[4][NQ];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int py = 0; py < NQ; ++py) SF[c][py] = SG[c][py] = 0.0;
 #pragma unroll
             for (int b = 0; b < NQ; ++b) {
-                double U[4] = {0.0, 0.0, 0.0, 0.0};
+                double U[4];
 #pragma unroll
-                for (int m = 0; m < NM; ++m)
+                for (int c = 0; c < 4; ++c) {
+                    double acc = 0.0;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) U[c] += Q[c][m] * T.phi[a][b][m];
+                    for (int py = 0; py < NQ; ++py) acc += Y[c][py] * T.lq[py][b];
+                    U[c] = acc;
+                }
                 double F[4], G[4];
                 euler_fg(U, A.gamma, F, G);
-                const double ww = T.ww[a][b];
 #pragma unroll
-                for (int m = 0; m < NM; ++m) {
-                    const double dx = T.dxi[a][b][m], dy = T.det[a][b][m];
+                for (int c = 0; c < 4; ++c)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) rhs[c][m] += ww * (F[c] * dx + G[c] * dy);
-                }
+                    for (int py = 0; py < NQ; ++py) {
+                        SF[c][py] += T.wl[py][b] * F[c];
+                        SG[c][py] += T.wdl[py][b] * G[c];
+                    }
             }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int px = 0; px < NQ; ++px)
+#pragma unroll
+                    for (int py = 0; py < NQ; ++py)
+                        rhs[c][px * NQ + py] += T.wdl[px][a] * SF[c][py] + T.wl[px][a] * SG[c][py];
+        }
         // face terms: right (x face j+1), left (x face j), down (y face i+1), up (y face i)
         const size_t fr = (size_t)ii * (A.nxu + 1) + (jj + 1), fl = fr - 1;
         const size_t fd = (size_t)(ii + 1) * A.nxu + jj, fu = (size_t)ii * A.nxu + jj;
@@ -405,29 +479,14 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
     if (A.limiter == ZK_LIMITER_PERSSON_ZS) {
         if (A.bct[k] == 0) persson_cell<P>(Q);
         zs_cell<P>(Q, A.gamma);
+    } else if (A.means) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) A.means[(size_t)c * np + k] = Q[c][0];
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int m = 0; m < NM; ++m) A.Qout[(size_t)(c * NM + m) * np + k] = Q[c][m];
-}
-
-// Mode-0 planes of Q after the ghost fill, for the TVB neighbour means.
-template <int P>
-__global__ void dg_means_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
-                                double *__restrict__ M, const StepScalars *__restrict__ S)
-{
-    constexpr int NM = (P + 1) * (P + 1);
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= np || S->done) return;
-    const int32_t e = src[k];
-    const size_t s = (size_t)(e >> kSrcShift);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        double v = Q[(size_t)(c * NM) * np + s];
-        if ((c == 1 && (e & 1)) || (c == 2 && (e & 2))) v = -v;
-        M[(size_t)c * np + k] = v;
-    }
 }
 
 __device__ __forceinline__ double minmod3(double a, double b, double c)
@@ -437,10 +496,24 @@ __device__ __forceinline__ double minmod3(double a, double b, double c)
     return 0.0;
 }
 
+// Mean of variable c of cell n after the ghost fill: the stage mean of its
+// source cell, negated for the normal momentum of slip ghosts.
+__device__ __forceinline__ double filled_mean(const double *__restrict__ M, const int32_t *__restrict__ src,
+                                              size_t np, size_t n, int c)
+{
+    const int32_t e = src[n];
+    double v = M[(size_t)c * np + (size_t)(e >> kSrcShift)];
+    if ((c == 1 && (e & 1)) || (c == 2 && (e & 2))) v = -v;
+    return v;
+}
+
 // Component-wise TVB limiter (Cockburn & Shu 2001, modified minmod with
 // constant M) then the positivity scaling, in place on the stage output.
+// Neighbour means are the combined stage means after the ghost fill; the cell
+// is written back only when the limiter or the scaling changed it.
 template <int P>
 __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, const double *__restrict__ M,
+                                                     const int32_t *__restrict__ src,
                                                      const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                      int ny, int nx, size_t np, double Mh2, double gamma,
                                                      const StepScalars *__restrict__ S)
@@ -461,11 +534,13 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
         constexpr int mxm = NQ, mym = 1;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double *a0 = M + (size_t)c * np;
-            const double ub = a0[k];
+            const double ub = C[c][0];
             const double ax = C[c][mxm], ay = C[c][mym];
-            const double axl = (fabs(ax) <= Mh2) ? ax : minmod3(ax, a0[k + 1] - ub, ub - a0[k - 1]);
-            const double ayl = (fabs(ay) <= Mh2) ? ay : minmod3(ay, a0[k + nx] - ub, ub - a0[k - nx]);
+            double axl = ax, ayl = ay;
+            if (!(fabs(ax) <= Mh2))
+                axl = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
+            if (!(fabs(ay) <= Mh2))
+                ayl = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
             if (axl != ax || ayl != ay) {
 #pragma unroll
                 for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
@@ -475,12 +550,14 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
             }
         }
     }
-    zs_cell<P>(C, gamma);
+    const bool changed = zs_cell<P>(C, gamma);
     flags[k] = (uint8_t)flag;
+    if (flag || changed) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+    }
 }
 
 // max(|u| + c) at the four corners of every bctype-0 cell, floors 1e-12
@@ -521,7 +598,7 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
 __global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__restrict__ Qin,
                                double *__restrict__ Qout, const int32_t *__restrict__ src,
                                const int32_t *__restrict__ ring, int nring, size_t np, int nm, double ca,
-                               double cb, int stage, const StepScalars *__restrict__ S)
+                               double cb, int stage, const StepScalars *__restrict__ S, double *__restrict__ M)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nring || S->done) return;
@@ -533,7 +610,139 @@ __global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__re
         const double qn = Qn[p * np + k];
         double qf = Qin[p * np + s];
         if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
-        Qout[p * np + k] = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
+        const double out = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
+        Qout[p * np + k] = out;
+        if (M && p % nm == 0) M[(size_t)var * np + k] = out;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// P = 3 cell kernel on FP64 matrix cores.
+//
+// One wave updates 16 cells.  With v_mfma_f64_16x16x4f64 (A: lane l holds
+// A[l%16][l/16]; B: B[l/16][l%16]; D: lane l holds D[l/16 + 4v][l%16], v=0..3)
+// the modal-to-point evaluation U_c(16 points x 16 cells) = Phi(16x16) Q_c(16x16)
+// and the projection rhs_c = W_F F_c + W_G G_c are 16x16x16 products.  Lane
+// group g = l/16 of column j = l%16 holds, for cell j, the modes (px = v, py = g)
+// of Q and rhs and the points (xi_a = v, eta_b = g) of U, F and G, v = 0..3, so
+// the flux at a point, the face terms and the Shu-Osher combination need no
+// data movement between lanes.  Points are numbered p = 4a + b, modes m = 4px + py.
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct MfmaTabs3 {
+    double phiA[4][64];  // K-step s: Phi[p = l%16][m = 4s + l/16]
+    double wfA[4][64];   // K-step s: w_a w_b L'_px(xi_a) L_py(eta_b) at m = l%16, p = 4s + l/16
+    double wgA[4][64];   // K-step s: w_a w_b L_px(xi_a) L'_py(eta_b)
+};
+__constant__ MfmaTabs3 c_mf3;
+
+MfmaTabs3 make_mfma_tables3()
+{
+    MfmaTabs3 M{};
+    const double *gp = GLP_H[4], *gw = GLW_H[4];
+    for (int s = 0; s < 4; ++s)
+        for (int l = 0; l < 64; ++l) {
+            const int i = l % 16, k = l / 16;
+            {   // evaluation: point i = 4a + b, mode 4s + k = (px = s, py = k)
+                const int a = i / 4, b = i % 4, px = s, py = k;
+                M.phiA[s][l] = legP_h(px, gp[a]) * legP_h(py, gp[b]);
+            }
+            {   // projection: mode i = (px = i/4, py = i%4), point 4s + k = (a = s, b = k)
+                const int px = i / 4, py = i % 4, a = s, b = k;
+                const double ww = gw[a] * gw[b];
+                M.wfA[s][l] = ww * legD_h(px, gp[a]) * legP_h(py, gp[b]);
+                M.wgA[s][l] = ww * legP_h(px, gp[a]) * legD_h(py, gp[b]);
+            }
+        }
+    return M;
+}
+
+template <int MINW>
+__global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
+{
+    constexpr int NQ = 4, NM = 16;
+    const DgTables &T = c_tab[2];
+    if (A.S->done) return;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, col = lane & 15;
+    const size_t ncell = (size_t)A.nxu * A.nyu;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (t0 >= ncell) return;  // whole wave past the end
+    const size_t t = t0 + col;
+    const bool live = t < ncell;
+    const size_t tc = live ? t : ncell - 1;   // padded lanes compute on a valid cell, store nothing
+    const int ii = (int)(tc / A.nxu), jj = (int)(tc % A.nxu);
+    const size_t k = (size_t)(ii + 3) * A.nx + (jj + 3);
+    const size_t np = A.np;
+    const bool upd = A.upd[k] != 0;
+
+    // this lane's modes (px = v, py = g) of the filled input cell
+    const int32_t e = A.src[k];
+    const size_t sk = (size_t)(e >> kSrcShift);
+    double Qr[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Qr[c][v] = A.Qin[(size_t)(c * NM + 4 * v + g) * np + sk];
+    if (e & 1)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Qr[1][v] = -Qr[1][v];
+    if (e & 2)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Qr[2][v] = -Qr[2][v];
+
+    // point values U_c at (xi_a = v, eta_b = g)
+    d4 U[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(c_mf3.phiA[s][lane], Qr[c][s], acc, 0, 0, 0);
+        U[c] = acc;
+    }
+    // fluxes point by point, projected as soon as they exist
+    d4 R[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) R[c] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double Up[4] = {U[0][v], U[1][v], U[2][v], U[3][v]};
+        double F[4], G[4];
+        euler_fg(Up, A.gamma, F, G);
+        const double wf = c_mf3.wfA[v][lane], wg = c_mf3.wgA[v][lane];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            R[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(wf, F[c], R[c], 0, 0, 0);
+            R[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(wg, G[c], R[c], 0, 0, 0);
+        }
+    }
+    // face terms: x faces project onto py (= g), y faces onto px (= v)
+    const size_t fr = (size_t)ii * (A.nxu + 1) + (jj + 1), fl = fr - 1;
+    const size_t fd = (size_t)(ii + 1) * A.nxu + jj, fu = (size_t)ii * A.nxu + jj;
+    const double inv2h = 0.5 / A.da;
+    const double dt = A.S->dt;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double gR = A.gx[(size_t)(c * NQ + g) * A.nfx + fr];
+        const double gL = A.gx[(size_t)(c * NQ + g) * A.nfx + fl];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double gD = A.gy[(size_t)(c * NQ + v) * A.nfy + fd];
+            const double gU = A.gy[(size_t)(c * NQ + v) * A.nfy + fu];
+            double rhs = R[c][v] + (T.sgn[v] * gL - gR) + (T.sgn[g] * gU - gD);
+            if (!upd) rhs = 0.0;
+            const int m = 4 * v + g;
+            const double dU = rhs * (T.invm[m] * inv2h);
+            const size_t idx = (size_t)(c * NM + m) * np + k;
+            const double qn = A.Qn[idx];
+            const double out = (A.stage == 0) ? qn + dt * dU : A.ca * qn + A.cb * (Qr[c][v] + dt * dU);
+            if (live) {
+                A.Qout[idx] = out;
+                if (A.means && m == 0) A.means[(size_t)c * np + k] = out;
+            }
+        }
     }
 }
 
@@ -546,6 +755,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.Qn = Qn; A.Qin = Qin; A.Qout = Qout;
     A.src = H.src; A.upd = H.upd; A.bct = H.bct; A.S = H.scal;
     A.gx = H.gx; A.gy = H.gy;
+    A.means = (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) ? H.means : nullptr;
     A.ny = H.g.ny; A.nx = H.g.nx; A.nxu = H.g.nx - 6; A.nyu = H.g.ny - 6;
     A.np = H.g.np;
     A.nfx = (size_t)(A.nxu + 1) * A.nyu;
@@ -567,22 +777,26 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             hipLaunchKernelGGL((dg_face_kernel<P, false>), dim3((A.nfy + 127) / 128), dim3(128), 0, H.stream, A);
         }
         ProfScope ps(H, "dg_cell");
-        hipLaunchKernelGGL((dg_cell_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, A);
+        if (P == 3 && H.cfg.limiter != ZK_LIMITER_PERSSON_ZS && !H.no_mfma) {
+            if (H.tune == 4)
+                hipLaunchKernelGGL(dg_cell_mfma3_kernel<4>, dim3((ncell + 63) / 64), dim3(256), 0, H.stream, A);
+            else if (H.tune == 3)
+                hipLaunchKernelGGL(dg_cell_mfma3_kernel<3>, dim3((ncell + 63) / 64), dim3(256), 0, H.stream, A);
+            else
+                hipLaunchKernelGGL(dg_cell_mfma3_kernel<2>, dim3((ncell + 63) / 64), dim3(256), 0, H.stream, A);
+        }
+        else
+            hipLaunchKernelGGL((dg_cell_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, A);
     }
     if (H.nring > 0) {
         ProfScope ps(H, "dg_ring");
         hipLaunchKernelGGL(dg_ring_kernel, dim3((H.nring + 255) / 256), dim3(256), 0, H.stream, Qn, Qin, Qout,
-                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal);
+                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means);
     }
     if (ncell > 0 && (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED)) {
-        {
-            ProfScope ps(H, "dg_means");
-            hipLaunchKernelGGL((dg_means_kernel<P>), dim3((H.g.np + 255) / 256), dim3(256), 0, H.stream, Qout, H.src,
-                           H.g.np, H.means, H.scal);
-        }
         ProfScope ps(H, "dg_tvb");
         hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                           H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.cfg.tvb_M * H.g.da * H.g.da, H.g.gamma,
+                           H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.cfg.tvb_M * H.g.da * H.g.da, H.g.gamma,
                            H.scal);
     }
 }
@@ -594,6 +808,8 @@ void dg_upload_tables(int device)
     if (device >= 0 && device < 64 && g_tables_ready[device]) return;
     DgTables T[3] = {make_tables(1), make_tables(2), make_tables(3)};
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), T, sizeof(T), 0, hipMemcpyHostToDevice);
+    const MfmaTabs3 M = make_mfma_tables3();
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_mf3), &M, sizeof(M), 0, hipMemcpyHostToDevice);
     if (device >= 0 && device < 64) g_tables_ready[device] = true;
 }
 

@@ -13,6 +13,7 @@
 // src_WENO/fluid.cpp:230-317; the stage combination follows :468-493.
 #include "zk_internal.h"
 #include "zk_device.h"
+#include "fv_face.h"
 
 namespace zk {
 
@@ -35,10 +36,19 @@ struct FvArgs {
     double ca, cb;  // Uout = ca*Un + cb*(Uf + dU) for stages 2 and 3
     int stage;
     int flux;
+    int positivity;
     int tiles_x;
 };
 
-template <int TX, int TY>
+// Face positivity check in FP64 (an option with no reference counterpart).
+__device__ __forceinline__ bool face_positive(const double Q[4], double gamma)
+{
+    if (!(Q[0] >= 1e-12)) return false;
+    const double p = (gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+    return p >= 1e-12;
+}
+
+template <int TX, int TY, bool F32>
 __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
 {
     constexpr int SW = TX + 6, SH = TY + 6;
@@ -91,7 +101,24 @@ __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
             double w[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) w[k] = s_u[v][sr + k * stride_r][sc + k * stride_c];
-            weno5_f64(w[0], w[1], w[2], w[3], w[4], w[5], QL[v], QR[v]);
+            if (F32) {
+                // FP32 reconstruction of the deltas to the face's left cell
+                const double r = w[2];
+                float dl, dr;
+                weno5_pair_f32((float)(w[0] - r), (float)(w[1] - r), 0.0f, (float)(w[3] - r), (float)(w[4] - r),
+                               (float)(w[5] - r), dl, dr);
+                QL[v] = r + (double)dl;
+                QR[v] = r + (double)dr;
+            } else {
+                weno5_pair_f64(w[0], w[1], w[2], w[3], w[4], w[5], QL[v], QR[v]);
+            }
+        }
+        if (A.positivity && !(face_positive(QL, A.gamma) && face_positive(QR, A.gamma))) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                QL[v] = s_u[v][sr + 2 * stride_r][sc + 2 * stride_c];
+                QR[v] = s_u[v][sr + 3 * stride_r][sc + 3 * stride_c];
+            }
         }
         // y faces: normal momentum is rho*v (component 2); the flux is called
         // with components 1 and 2 swapped and swapped back (fluid.cpp:444-451).
@@ -100,7 +127,8 @@ __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
         L[1] = isx ? QL[1] : QL[2]; L[2] = isx ? QL[2] : QL[1];
         R[1] = isx ? QR[1] : QR[2]; R[2] = isx ? QR[2] : QR[1];
         if (A.flux == ZK_FLUX_HLL) hll_f64(L, R, A.gamma, F);
-        else hllc_fv_f64(L, R, A.gamma, F);
+        else if (F32) hllc_fv_f32(L, R, A.gamma, F);
+        else hllc_fv_fast_f64(L, R, A.gamma, F);
         const double F1 = isx ? F[1] : F[2];
         const double F2 = isx ? F[2] : F[1];
         if (isx) {
@@ -188,13 +216,18 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     A.ca = stage == 1 ? 0.75 : (1.0 / 3.0);
     A.cb = stage == 1 ? 0.25 : (2.0 / 3.0);
     A.flux = H.cfg.flux;
+    A.positivity = H.cfg.positivity;
     const int ux = H.g.nx - 6, uy = H.g.ny - 6;
     A.tiles_x = (ux + FV_TX - 1) / FV_TX;
     const int tiles_y = (uy + FV_TY - 1) / FV_TY;
     if (ux > 0 && uy > 0) {
         ProfScope ps(H, "fv_stage");
-        hipLaunchKernelGGL((fv_stage_kernel<FV_TX, FV_TY>), dim3(A.tiles_x * tiles_y), dim3(FV_THREADS), 0,
-                           H.stream, A);
+        if (H.cfg.precision == ZK_PREC_FP32_BULK)
+            hipLaunchKernelGGL((fv_stage_kernel<FV_TX, FV_TY, true>), dim3(A.tiles_x * tiles_y), dim3(FV_THREADS),
+                               0, H.stream, A);
+        else
+            hipLaunchKernelGGL((fv_stage_kernel<FV_TX, FV_TY, false>), dim3(A.tiles_x * tiles_y), dim3(FV_THREADS),
+                               0, H.stream, A);
     }
     if (H.nring > 0) {
         ProfScope ps(H, "fv_ring");

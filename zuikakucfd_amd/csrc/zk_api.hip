@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -196,6 +197,8 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     Handle *H = new Handle();
     H->cfg = *cfg;
     H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
+    if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
+    if (const char *v = std::getenv("ZK_TUNE")) H->tune = std::atoi(v);
     e = hipSetDevice(cfg->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

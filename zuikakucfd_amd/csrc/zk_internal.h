@@ -41,6 +41,8 @@ struct Handle {
     Grid g;
     hipStream_t stream = nullptr;
     bool loaded = false;
+    bool no_mfma = false;     // force the VALU cell kernel (ZK_NO_MFMA=1), for comparison
+    int tune = 0;             // kernel variant selector (ZK_TUNE), used by the autotuner
     int64_t n_updated = 0;
 
     // device buffers

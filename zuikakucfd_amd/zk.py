@@ -41,6 +41,7 @@ class ZkConfig(C.Structure):
         ("limiter", C.c_int),
         ("tvb_M", C.c_double),
         ("device", C.c_int),
+        ("positivity", C.c_int),
     ]
 
 
@@ -101,9 +102,9 @@ class Solver:
 
     def __init__(self, method: int = METHOD_WENO, P: int = 1, precision: int = PREC_FP64,
                  flux: int = FLUX_HLLC, limiter: int = LIMITER_PERSSON_ZS, tvb_M: float = 50.0,
-                 device: int = 0):
+                 device: int = 0, positivity: bool = False):
         self.lib = load_library()
-        self.cfg = ZkConfig(method, P, precision, flux, limiter, float(tvb_M), device)
+        self.cfg = ZkConfig(method, P, precision, flux, limiter, float(tvb_M), device, 1 if positivity else 0)
         h = C.c_void_p()
         st = self.lib.zk_create(C.byref(self.cfg), C.byref(h))
         if st != ZK_OK:
