@@ -106,6 +106,136 @@ def cpu_baseline(w: dict, m, dt: float, budget_s: float = 20.0) -> dict:
                       f"({threads} threads, {model}), {el:.1f} s"}
 
 
+def _rel_l1(a, b) -> float:
+    import numpy as np
+
+    return float(np.abs(a - b).sum() / np.abs(b).sum())
+
+
+def path_weno_riemann256(steps: int) -> dict:
+    """Config 1: plain FP64 WENO5 on the shipped 256^2 four-quadrant input."""
+    from zuikakucfd_amd import meshio, zk
+
+    m = meshio.riemann_2d(256)
+    s = zk.Solver(method=zk.METHOD_WENO)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    dt = 0.5 * m.da / s.max_speed()
+    s.step(dt, 3)
+    ms = s.step_timed(dt, steps)
+    n = s.updated_cells()
+    s.close()
+    rate = 3 * n * steps / (ms / 1e3)
+    return {"config": "WENO5 FP64, 2D-Riemann 256^2, CFL 0.5", "cell_stage_per_s": rate,
+            "hbm_frac": rate * algo_bytes_per_cell_stage(1) / (HBM_PEAK_GBS * 1e9), "steps": steps}
+
+
+def path_jet(steps: int) -> dict:
+    """Config 3: Mach-80 jet 2048x1024, gamma 5/3, CFL 0.25; FP32-bulk WENO/HLLC
+    against the plain FP64 path (both with the face positivity check, without
+    which the reference scheme loses positivity on this jet), error vs FP64."""
+    from zuikakucfd_amd import meshio, zk
+
+    m = meshio.jet()
+    out = {"config": "WENO5 jet 2048x1024, gamma 5/3, CFL 0.25, positivity check", "steps": steps}
+    res = {}
+    for name, prec in (("fp64", zk.PREC_FP64), ("fp32_bulk", zk.PREC_FP32_BULK)):
+        s = zk.Solver(method=zk.METHOD_WENO, precision=prec, positivity=True)
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+        if name == "fp64":
+            dt = 0.25 * m.da / s.max_speed()
+        ms = s.step_timed(dt, steps)
+        n = s.updated_cells()
+        res[name] = (3 * n * steps / (ms / 1e3), s.get_conserved())
+        s.close()
+    out["fp64_cell_stage_per_s"] = res["fp64"][0]
+    out["fp32_bulk_cell_stage_per_s"] = res["fp32_bulk"][0]
+    out["speedup"] = res["fp32_bulk"][0] / res["fp64"][0]
+    out["l1_rho_vs_fp64"] = _rel_l1(res["fp32_bulk"][1][0], res["fp64"][1][0])
+    return out
+
+
+def path_ensemble(steps: int, ninst: int = 256) -> dict:
+    """Config 5: 256 seeded four-quadrant problems, N uniform over
+    {128,192,256,384,512}, per-instance CFL 0.5 dt; FP32-bulk batched ensemble
+    (persistent launch, graph replay, autotuned tiles) against the plain FP64
+    per-instance loop; density error vs the FP64 loop."""
+    import numpy as np
+
+    from zuikakucfd_amd import autotune, meshio, zk
+
+    rng = np.random.default_rng(2024)
+    sizes = rng.choice([128, 192, 256, 384, 512], size=ninst)
+    ms = [meshio.random_riemann(1000 + k, int(n)) for k, n in enumerate(sizes)]
+
+    def make(prec):
+        E = zk.Ensemble(precision=prec, positivity=True)
+        for m in ms:
+            E.add(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype, cfl=0.5)
+        E.finalize()
+        return E
+
+    tile, tune_ms = autotune.tune_ensemble(lambda: make(zk.PREC_FP32_BULK),
+                                           key=f"ens-fp32-{ninst}-{int(sum(int(n) ** 2 for n in sizes))}")
+    E = make(zk.PREC_FP32_BULK)
+    E.set_tuning(tile=tile, use_graph=True)
+    _, ncell = E.size()
+    E.step(1)
+    E.sync()
+    E2 = make(zk.PREC_FP32_BULK)
+    E2.set_tuning(tile=tile, use_graph=True)
+    t0 = time.perf_counter()
+    E2.step(steps)
+    E2.sync()
+    t_ens = time.perf_counter() - t0
+    del E
+    # plain FP64 per-instance loop: one handle per problem, on-device CFL stepping
+    solvers = []
+    for m in ms:
+        s = zk.Solver(method=zk.METHOD_WENO, positivity=True)
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+        solvers.append(s)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step_cfl(0.5, 1e30, steps)
+    t_loop = time.perf_counter() - t0
+    errs = []
+    nonfinite = 0
+    for k in range(ninst):
+        U, _ = E2.get(k)
+        if not np.isfinite(U).all():
+            nonfinite += 1
+            continue
+        errs.append(_rel_l1(U[0], solvers[k].get_conserved()[0]))
+    for s in solvers:
+        s.close()
+    E2.close()
+    r_ens = 3 * ncell * steps / t_ens
+    r_loop = 3 * ncell * steps / t_loop
+    return {"config": f"{ninst} WENO5 four-quadrant problems, N in {{128..512}}, {ncell} updated cells, CFL 0.5",
+            "steps": steps, "fp32_batched_cell_stage_per_s": r_ens, "fp64_loop_cell_stage_per_s": r_loop,
+            "speedup": r_ens / r_loop, "l1_rho_vs_fp64_max": max(errs), "nonfinite_instances": nonfinite,
+            "positivity_check": True, "tile": tile, "autotune_ms": tune_ms,
+            "timing": "wall clock around the launches with a device sync at each end"}
+
+
+def device_copy(nbytes: int) -> int:
+    """One device-to-device hipMemcpy of nbytes (the runtime's plain copy
+    kernel), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,7 +243,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dg3_tvb_riemann1024", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-paths", action="store_true", help="skip the other configurations")
+    ap.add_argument("--path-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB on the device (a plain copy "
+                         "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -168,8 +303,14 @@ def main() -> int:
         "kernels": rep,
         "finite": finite,
     }
+    if not args.no_paths:
+        line["paths"] = {"weno_fp64_riemann256": path_weno_riemann256(4 * args.path_steps),
+                         "jet_fp32_bulk": path_jet(args.path_steps),
+                         "ensemble256": path_ensemble(args.path_steps)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, m, dt)
+    if args.calibrate_copy_mib > 0:
+        line["calibration_copy_bytes"] = device_copy(args.calibrate_copy_mib << 20)
     print(json.dumps(line))
     return 0
 

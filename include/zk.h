@@ -136,6 +136,40 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len);
  * the loop domain of src_WENO/fluid.cpp:398-402). */
 zk_status zk_updated_cells(zk_handle *h, int64_t *out);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles of independent WENO5 problems (no reference counterpart:
+ * the reference runs one problem per MPI job).  Instances of different grid
+ * sizes are packed and advanced together, each with its own
+ * dt = cfl * da / lambda_max recomputed on the device every step (the
+ * CFL-correct rule, lambda as computeMaxSpeed, src_WENO/fluid.cpp:539-553).
+ * One step is a fixed sequence of launches captured once as a HIP graph.
+ * ------------------------------------------------------------------------- */
+typedef struct zk_ensemble zk_ensemble;
+
+zk_status zk_ensemble_create(const zk_config *cfg, zk_ensemble **out);
+/* Append one instance (primitive fields, as zk_load); copied. */
+zk_status zk_ensemble_add(zk_ensemble *e, int ny, int nx, double da, double gamma,
+                          const double *rho, const double *u, const double *v,
+                          const double *p, const int32_t *bctype, double cfl);
+/* Upload every instance; no zk_ensemble_add afterwards. */
+zk_status zk_ensemble_finalize(zk_ensemble *e);
+/* tile: 1 = 16x15, 2 = 32x15 cells per work item; use_graph: replay a captured step. */
+zk_status zk_ensemble_set_tuning(zk_ensemble *e, int tile, int use_graph);
+/* Advance every instance nsteps steps (asynchronous). */
+zk_status zk_ensemble_step(zk_ensemble *e, int nsteps);
+zk_status zk_ensemble_sync(zk_ensemble *e);
+zk_status zk_ensemble_size(zk_ensemble *e, int *ninst, int64_t *updated_cells);
+/* Conserved planes (4*ny*nx) and time reached of instance idx. */
+zk_status zk_ensemble_get(zk_ensemble *e, int idx, double *U, double *t_out);
+const char *zk_ensemble_last_error(const zk_ensemble *e);
+zk_status zk_ensemble_destroy(zk_ensemble *e);
+
+/* Kernel-variant selection for the autotuner (no reference counterpart):
+ * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
+ * default, 2/3/4 = minimum waves per SIMD of the P=3 MFMA cell kernel,
+ * -1 = VALU cell kernel. */
+zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,17 @@ void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, bool slip, s
         }
 }
 
+}  // namespace
+
+namespace zk {
+void build_src_map_public(const std::vector<int32_t> &bct, int ny, int nx, bool slip, std::vector<int32_t> &src)
+{
+    build_src_map(bct, ny, nx, slip, src);
+}
+}  // namespace zk
+
+namespace {
+
 zk_status enqueue_step(Handle *H)
 {
     if (H->cfg.method == ZK_METHOD_WENO) {
@@ -536,6 +547,20 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len)
         return ZK_ERR_ARG;
     }
     std::memcpy(buf, js.c_str(), js.size() + 1);
+    return ZK_OK;
+}
+
+zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    if (fv_tile < 0 || fv_tile > 2 || dg_variant < -1 || dg_variant > 4) {
+        H->err = "zk_set_tuning: fv_tile in 0..2, dg_variant in -1..4";
+        return ZK_ERR_ARG;
+    }
+    H->fv_tile = fv_tile;
+    H->no_mfma = dg_variant == -1;
+    H->tune = dg_variant > 0 ? dg_variant : 0;
     return ZK_OK;
 }
 

@@ -28,7 +28,10 @@ _ERR_NAMES = {1: "ZK_ERR_ARG", 2: "ZK_ERR_HIP", 3: "ZK_ERR_NUMERIC", 4: "ZK_ERR_
 EXPORTED = [
     "zk_create", "zk_destroy", "zk_last_error", "zk_load", "zk_set_state", "zk_get_state",
     "zk_get_conserved", "zk_max_speed", "zk_step", "zk_step_cfl", "zk_sync", "zk_step_timed",
-    "zk_get_flags", "zk_updated_cells", "zk_profile", "zk_profile_report",
+    "zk_get_flags", "zk_updated_cells", "zk_profile", "zk_profile_report", "zk_set_tuning",
+    "zk_ensemble_create", "zk_ensemble_add", "zk_ensemble_finalize", "zk_ensemble_set_tuning",
+    "zk_ensemble_step", "zk_ensemble_sync", "zk_ensemble_size", "zk_ensemble_get",
+    "zk_ensemble_last_error", "zk_ensemble_destroy",
 ]
 
 
@@ -81,8 +84,20 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_updated_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.zk_profile.argtypes = [vp, C.c_int]
     lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
+    lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.zk_ensemble_create.argtypes = [C.POINTER(ZkConfig), C.POINTER(vp)]
+    lib.zk_ensemble_add.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, ip, C.c_double]
+    lib.zk_ensemble_finalize.argtypes = [vp]
+    lib.zk_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.zk_ensemble_step.argtypes = [vp, C.c_int]
+    lib.zk_ensemble_sync.argtypes = [vp]
+    lib.zk_ensemble_size.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    lib.zk_ensemble_get.argtypes = [vp, C.c_int, dp, dp]
+    lib.zk_ensemble_last_error.argtypes = [vp]
+    lib.zk_ensemble_last_error.restype = C.c_char_p
+    lib.zk_ensemble_destroy.argtypes = [vp]
     for name in EXPORTED:
-        if name != "zk_last_error":
+        if name not in ("zk_last_error", "zk_ensemble_last_error"):
             getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib
@@ -196,6 +211,9 @@ class Solver:
         self._check(self.lib.zk_step_timed(self.h, float(dt), int(nsteps), C.byref(ms)), "zk_step_timed")
         return ms.value
 
+    def set_tuning(self, fv_tile: int = 0, dg_variant: int = 0) -> None:
+        self._check(self.lib.zk_set_tuning(self.h, int(fv_tile), int(dg_variant)), "zk_set_tuning")
+
     def profile(self, enable: bool = True) -> None:
         self._check(self.lib.zk_profile(self.h, 1 if enable else 0), "zk_profile")
 
@@ -208,3 +226,68 @@ class Solver:
 
     def sync(self) -> None:
         self._check(self.lib.zk_sync(self.h), "zk_sync")
+
+
+class Ensemble:
+    """Independent WENO5 problems of uneven sizes advanced together on one GPU,
+    each with its own on-device CFL time step (zk_ensemble_* in include/zk.h)."""
+
+    def __init__(self, precision: int = PREC_FP64, flux: int = FLUX_HLLC, positivity: bool = False,
+                 device: int = 0):
+        self.lib = load_library()
+        self.cfg = ZkConfig(METHOD_WENO, 0, precision, flux, 0, 0.0, device, 1 if positivity else 0)
+        h = C.c_void_p()
+        st = self.lib.zk_ensemble_create(C.byref(self.cfg), C.byref(h))
+        if st != ZK_OK:
+            raise ZkError(f"zk_ensemble_create failed ({_ERR_NAMES.get(st, st)})")
+        self.h = h
+        self.shapes = []
+
+    def _check(self, st: int, what: str) -> None:
+        if st != ZK_OK:
+            raise ZkError(f"{what}: {_ERR_NAMES.get(st, st)}: {self.lib.zk_ensemble_last_error(self.h).decode()}")
+
+    def add(self, da, gamma, rho, u, v, p, bctype, cfl: float) -> int:
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        ny, nx = rho.shape
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(ny, nx) for a in (u, v, p)]
+        bc = np.ascontiguousarray(bctype, dtype=np.int32).reshape(ny, nx)
+        self._check(self.lib.zk_ensemble_add(self.h, ny, nx, float(da), float(gamma), _dptr(rho), *map(_dptr, arrs),
+                                             bc.ctypes.data_as(C.POINTER(C.c_int32)), float(cfl)), "zk_ensemble_add")
+        self.shapes.append((ny, nx))
+        return len(self.shapes) - 1
+
+    def finalize(self) -> None:
+        self._check(self.lib.zk_ensemble_finalize(self.h), "zk_ensemble_finalize")
+
+    def set_tuning(self, tile: int = 1, use_graph: bool = True) -> None:
+        self._check(self.lib.zk_ensemble_set_tuning(self.h, int(tile), 1 if use_graph else 0), "zk_ensemble_set_tuning")
+
+    def step(self, nsteps: int) -> None:
+        self._check(self.lib.zk_ensemble_step(self.h, int(nsteps)), "zk_ensemble_step")
+
+    def sync(self) -> None:
+        self._check(self.lib.zk_ensemble_sync(self.h), "zk_ensemble_sync")
+
+    def size(self) -> tuple[int, int]:
+        n, c = C.c_int(), C.c_int64()
+        self._check(self.lib.zk_ensemble_size(self.h, C.byref(n), C.byref(c)), "zk_ensemble_size")
+        return n.value, c.value
+
+    def get(self, idx: int) -> tuple[np.ndarray, float]:
+        ny, nx = self.shapes[idx]
+        U = np.empty((4, ny, nx), dtype=np.float64)
+        t = C.c_double()
+        self._check(self.lib.zk_ensemble_get(self.h, int(idx), _dptr(U), C.byref(t)), "zk_ensemble_get")
+        return U, t.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.zk_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
