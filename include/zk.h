@@ -170,6 +170,47 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
  * -1 = VALU cell kernel. */
 zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant);
 
+/* ---------------------------------------------------------------------------
+ * Learned troubled-cell indicator for DG P=2 (no reference counterpart).
+ * A 192-64-64-1 MLP (BF16 weights on v_mfma_f32_16x16x32_bf16, FP32
+ * accumulation, FP32 Adam master weights) scores every cell from its own and
+ * its four neighbours' normalised modes.  Training data are (features, TVB
+ * flag) pairs harvested from the solver's own TVB runs on the device.
+ * ------------------------------------------------------------------------- */
+typedef struct zk_mlp zk_mlp;
+
+/* He-initialised network (seeded) with room for `capacity` harvested samples. */
+zk_status zk_mlp_create(int device, uint64_t seed, int capacity, zk_mlp **out);
+zk_status zk_mlp_destroy(zk_mlp *m);
+const char *zk_mlp_last_error(const zk_mlp *m);
+/* Number of FP32 parameters: W1[64][192], b1[64], W2[64][64], b2[64], w3[64], b3. */
+int zk_mlp_param_count(void);
+zk_status zk_mlp_get_params(zk_mlp *m, float *w);
+zk_status zk_mlp_set_params(zk_mlp *m, const float *w);
+/* Loss and parameter gradients of n host samples (n % 32 == 0); X holds BF16
+ * bit patterns [n][192]; binary cross-entropy on the sigmoid output. */
+zk_status zk_mlp_grad(zk_mlp *m, const uint16_t *X, const float *y, int n, float *grad, float *loss);
+/* Scores of n host samples (n % 16 == 0). */
+zk_status zk_mlp_infer(zk_mlp *m, const uint16_t *X, int n, float *scores);
+/* Copy out up to max harvested samples; *count receives the number stored. */
+zk_status zk_mlp_dataset(zk_mlp *m, int *count, uint16_t *X, float *y, int max);
+zk_status zk_mlp_clear_dataset(zk_mlp *m);
+/* Adam over the harvested set: epochs x (count / B) steps of batch B (B % 32 == 0). */
+zk_status zk_mlp_train(zk_mlp *m, int epochs, int B, float lr, uint64_t seed, float *loss_per_epoch);
+/* Harvest (features, TVB flag) pairs during the stages of a DG P=2 TVB handle;
+ * positives kept with probability keep_pos, negatives keep_neg; m = NULL stops. */
+zk_status zk_harvest(zk_handle *h, zk_mlp *m, float keep_pos, float keep_neg);
+/* Use m in a ZK_LIMITER_LEARNED handle: scores above hi mark a troubled cell,
+ * below lo a smooth one; scores in [lo, hi] and a random `audit` fraction of
+ * cells fall back to the TVB test, which also scores the network. */
+zk_status zk_attach_mlp(zk_handle *h, zk_mlp *m, float lo, float hi, float audit);
+/* Counters since the last reset: audited, agreed, missed (TVB troubled, net
+ * not), false flags (net troubled, TVB not), net-flagged, in-band. */
+zk_status zk_learned_stats(zk_handle *h, int64_t out[6], int reset);
+/* Features (BF16 bits, [n][192]) and, when m is given, scores of n cells of the
+ * current state (cells as row-major indices inside [3,ny-3)x[3,nx-3)). */
+zk_status zk_mlp_features(zk_handle *h, zk_mlp *m, const int32_t *cells, int n, uint16_t *X, float *scores);
+
 #ifdef __cplusplus
 }
 #endif

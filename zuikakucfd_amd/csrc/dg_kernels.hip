@@ -18,8 +18,17 @@
 // kernel after the combination, followed by the positivity scaling.
 #include "zk_internal.h"
 #include "zk_device.h"
+#include "mlp_device.h"
 
 namespace zk {
+// learned indicator hooks (mlp_kernels.hip)
+const __bf16 *mlp_w1b(const zk_mlp *M);
+const __bf16 *mlp_w2p(const zk_mlp *M);
+const float *mlp_w(const zk_mlp *M);
+__bf16 *mlp_X(zk_mlp *M);
+float *mlp_y(zk_mlp *M);
+int *mlp_count(zk_mlp *M);
+int mlp_cap(const zk_mlp *M);
 
 namespace {
 
@@ -746,6 +755,169 @@ __global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Learned troubled-cell indicator (P = 2)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ c * 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// The TVB test of one cell (per variable: modified minmod of the two linear
+// modes against the filled neighbour means); returns whether any variable
+// changes and leaves the limited slopes in axl/ayl.
+template <int P>
+__device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], const double *__restrict__ M,
+                                         const int32_t *__restrict__ src, size_t np, size_t k, int nx, double Mh2,
+                                         double axl[4], double ayl[4], bool chg[4])
+{
+    constexpr int NQ = P + 1;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double ub = C[c][0], ax = C[c][NQ], ay = C[c][1];
+        axl[c] = ax;
+        ayl[c] = ay;
+        if (!(fabs(ax) <= Mh2))
+            axl[c] = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
+        if (!(fabs(ay) <= Mh2))
+            ayl[c] = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
+        chg[c] = (axl[c] != ax || ayl[c] != ay);
+        any = any || chg[c];
+    }
+    return any;
+}
+
+// Scores of every updated-range cell (16 per wave, MFMA), written to score[k].
+__global__ __launch_bounds__(256) void dg_score_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src,
+                                                       int nx, int nxu, size_t ncell, size_t np,
+                                                       const __bf16 *__restrict__ w1b, const __bf16 *__restrict__ w2p,
+                                                       const float *__restrict__ w, float *__restrict__ score,
+                                                       const StepScalars *__restrict__ S)
+{
+    if (S->done) return;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (t0 >= ncell) return;
+    const int i = threadIdx.x & 15;
+    const size_t t = t0 + i < ncell ? t0 + i : ncell - 1;
+    const size_t k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+    const float sc = mlp_score16(Q, src, np, nx, k, w1b, w2p, w);
+    if ((threadIdx.x & 63) < 16 && t0 + i < ncell) score[k] = sc;
+}
+
+// Learned limiter: outside the uncertainty band [lo, hi] the score decides
+// (troubled = score > hi, then every variable is limited); inside the band and
+// on a random audit fraction the plain TVB test decides.  Audited cells also
+// record agreement of the network's 0.5 decision with the TVB test.  The
+// positivity scaling follows, as in the TVB limiter.
+template <int P>
+__global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q, const double *__restrict__ M,
+                                                         const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                                         uint8_t *__restrict__ flags, const float *__restrict__ score,
+                                                         int ny, int nx, size_t np, double Mh2, double gamma, float lo,
+                                                         float hi, uint32_t audit_thresh, uint32_t salt,
+                                                         unsigned long long *__restrict__ stats,
+                                                         const StepScalars *__restrict__ S)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nxu * nyu || S->done) return;
+    const int i = 3 + (int)(t / nxu), j = 3 + (int)(t % nxu);
+    const size_t k = (size_t)i * nx + j;
+    double C[4][NM];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+    int flag = 0;
+    if (bct[k] == 0) {
+        const float sc = score[k];
+        const bool audit = hash3((uint32_t)k, salt, 0x5bd1e995u) < audit_thresh;
+        const bool band = sc >= lo && sc <= hi;
+        double axl[4], ayl[4];
+        bool chg[4];
+        if (audit || band) {
+            const bool tv = tvb_test<P>(C, M, src, np, k, nx, Mh2, axl, ayl, chg);
+            if (audit) {
+                const bool net = sc > 0.5f;
+                atomicAdd(stats + 0, 1ull);
+                if (net == tv) atomicAdd(stats + 1, 1ull);
+                else if (tv) atomicAdd(stats + 2, 1ull);
+                else atomicAdd(stats + 3, 1ull);
+            }
+            if (band) atomicAdd(stats + 5, 1ull);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (chg[c]) {
+#pragma unroll
+                    for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
+                    C[c][NQ] = axl[c];
+                    C[c][1] = ayl[c];
+                    flag = 1;
+                }
+        } else if (sc > hi) {
+            tvb_test<P>(C, M, src, np, k, nx, Mh2, axl, ayl, chg);
+            atomicAdd(stats + 4, 1ull);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
+                C[c][NQ] = axl[c];
+                C[c][1] = ayl[c];
+            }
+            flag = 1;
+        }
+    }
+    const bool changed = zs_cell<P>(C, gamma);
+    flags[k] = (uint8_t)flag;
+    if (flag || changed) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+    }
+}
+
+// Harvest (features, TVB flag) pairs of the combined (pre-limit) stage state.
+// Positives are kept with probability keep_pos, negatives with keep_neg.
+template <int P>
+__global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restrict__ Q, const double *__restrict__ M,
+                                                         const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                                         int ny, int nx, size_t np, double Mh2, uint32_t keep_pos,
+                                                         uint32_t keep_neg, uint32_t salt, __bf16 *__restrict__ X,
+                                                         float *__restrict__ y, int *__restrict__ count, int cap,
+                                                         const StepScalars *__restrict__ S)
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nxu * nyu || S->done) return;
+    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+    if (bct[k] != 0) return;
+    double C[4][NM];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+    double axl[4], ayl[4];
+    bool chg[4];
+    const bool tv = tvb_test<P>(C, M, src, np, k, nx, Mh2, axl, ayl, chg);
+    if (hash3((uint32_t)k, salt, 0x27d4eb2fu) >= (tv ? keep_pos : keep_neg)) return;
+    const int slot = atomicAdd(count, 1);
+    if (slot >= cap) return;
+    double mean[4], scale[4];
+    mlp_scales(Q, np, k, mean, scale);
+    for (int f = 0; f < 192; ++f) X[(size_t)slot * 192 + f] = (__bf16)mlp_feature(Q, src, np, nx, k, mean, scale, f);
+    y[slot] = tv ? 1.0f : 0.0f;
+}
+
 bool g_tables_ready[64] = {false};
 
 template <int P>
@@ -793,11 +965,32 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         hipLaunchKernelGGL(dg_ring_kernel, dim3((H.nring + 255) / 256), dim3(256), 0, H.stream, Qn, Qin, Qout,
                            H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means);
     }
-    if (ncell > 0 && (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED)) {
+    const double Mh2 = H.cfg.tvb_M * H.g.da * H.g.da;
+    H.stage_counter += 1;
+    if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB && H.harvest && P == 2) {
+        ProfScope ps(H, "dg_harvest");
+        auto frac = [](float f) { return f >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(f * 4294967296.0); };
+        hipLaunchKernelGGL((dg_harvest_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout, H.means,
+                           H.src, H.bct, H.g.ny, H.g.nx, H.g.np, Mh2, frac(H.keep_pos), frac(H.keep_neg),
+                           H.stage_counter, mlp_X(H.harvest), mlp_y(H.harvest), mlp_count(H.harvest),
+                           mlp_cap(H.harvest), H.scal);
+    }
+    if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB) {
         ProfScope ps(H, "dg_tvb");
         hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                           H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.cfg.tvb_M * H.g.da * H.g.da, H.g.gamma,
-                           H.scal);
+                           H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
+    }
+    if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
+        {
+            ProfScope ps(H, "dg_score");
+            hipLaunchKernelGGL(dg_score_kernel, dim3((ncell + 63) / 64), dim3(256), 0, H.stream, Qout, H.src, H.g.nx,
+                               A.nxu, ncell, H.g.np, mlp_w1b(H.mlp), mlp_w2p(H.mlp), mlp_w(H.mlp), H.score, H.scal);
+        }
+        ProfScope ps(H, "dg_learned");
+        const uint32_t audit = H.mlp_audit >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(H.mlp_audit * 4294967296.0);
+        hipLaunchKernelGGL((dg_learned_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
+                           H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.mlp_lo,
+                           H.mlp_hi, audit, H.stage_counter, H.stats, H.scal);
     }
 }
 

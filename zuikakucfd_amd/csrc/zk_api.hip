@@ -17,6 +17,7 @@
 
 namespace zk {
 void dg_upload_tables(int device);
+zk_status mlp_features_of(Handle &H, zk_mlp *M, const int32_t *cells, int n, uint16_t *X, float *scores);
 }
 
 using zk::Handle;
@@ -80,13 +81,14 @@ zk_status check_args(Handle *H)
 void free_all(Handle *H)
 {
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
-                    H->scal, H->lam_bits, H->bct};
+                    H->scal, H->lam_bits, H->bct, H->stats, H->score};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
     H->src = nullptr; H->upd = nullptr; H->ring = nullptr;
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
     H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
+    H->stats = nullptr; H->score = nullptr;
     H->loaded = false;
 }
 
@@ -189,6 +191,10 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     }
     if (cfg->method == ZK_METHOD_DG && (cfg->limiter < 0 || cfg->limiter > ZK_LIMITER_LEARNED)) {
         g_create_err = "unknown limiter";
+        return ZK_ERR_ARG;
+    }
+    if (cfg->method == ZK_METHOD_DG && cfg->limiter == ZK_LIMITER_LEARNED && cfg->P != 2) {
+        g_create_err = "the learned troubled-cell indicator is defined for DG P=2";
         return ZK_ERR_ARG;
     }
     if (cfg->precision != ZK_PREC_FP64 && cfg->precision != ZK_PREC_FP32_BULK) {
@@ -302,6 +308,12 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         ZK_HIP(hipMalloc(&H->gx, 4 * (size_t)NQ * (nxu + 1) * nyu * sizeof(double)));
         ZK_HIP(hipMalloc(&H->gy, 4 * (size_t)NQ * nxu * (nyu + 1) * sizeof(double)));
         ZK_HIP(hipMalloc(&H->means, 4 * np * sizeof(double)));
+        ZK_HIP(hipMalloc(&H->stats, 8 * sizeof(unsigned long long)));
+        ZK_HIP(hipMemset(H->stats, 0, 8 * sizeof(unsigned long long)));
+        if (H->cfg.limiter == ZK_LIMITER_LEARNED) {
+            ZK_HIP(hipMalloc(&H->score, np * sizeof(float)));
+            ZK_HIP(hipMemset(H->score, 0, np * sizeof(float)));
+        }
     }
     ZK_HIP(hipMemcpy(H->q, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
     ZK_HIP(hipMemcpy(H->qa, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
@@ -398,6 +410,10 @@ zk_status zk_step(zk_handle *h, double dt, int nsteps)
     Handle *H = reinterpret_cast<Handle *>(h);
     zk_status st = check_args(H);
     if (st) return st;
+    if (H->cfg.method == ZK_METHOD_DG && H->cfg.limiter == ZK_LIMITER_LEARNED && !H->mlp) {
+        H->err = "ZK_LIMITER_LEARNED needs a network: call zk_attach_mlp first";
+        return ZK_ERR_ARG;
+    }
     if (!(dt >= 0.0) || nsteps < 0) {
         H->err = "zk_step: need dt >= 0 and nsteps >= 0";
         return ZK_ERR_ARG;
@@ -562,6 +578,82 @@ zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant)
     H->no_mfma = dg_variant == -1;
     H->tune = dg_variant > 0 ? dg_variant : 0;
     return ZK_OK;
+}
+
+zk_status zk_attach_mlp(zk_handle *h, zk_mlp *m, float lo, float hi, float audit)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    if (H->cfg.method != ZK_METHOD_DG || H->cfg.limiter != ZK_LIMITER_LEARNED) {
+        H->err = "zk_attach_mlp: handle must be DG P=2 with ZK_LIMITER_LEARNED";
+        return ZK_ERR_ARG;
+    }
+    if (!(lo >= 0.0f && lo <= hi && hi <= 1.0f && audit >= 0.0f && audit <= 1.0f)) {
+        H->err = "zk_attach_mlp: need 0 <= lo <= hi <= 1 and 0 <= audit <= 1";
+        return ZK_ERR_ARG;
+    }
+    H->mlp = m;
+    H->mlp_lo = lo;
+    H->mlp_hi = hi;
+    H->mlp_audit = audit;
+    return ZK_OK;
+}
+
+zk_status zk_harvest(zk_handle *h, zk_mlp *m, float keep_pos, float keep_neg)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    if (m && (H->cfg.method != ZK_METHOD_DG || H->cfg.P != 2 || H->cfg.limiter != ZK_LIMITER_TVB)) {
+        H->err = "zk_harvest: handle must be DG P=2 with ZK_LIMITER_TVB";
+        return ZK_ERR_ARG;
+    }
+    H->harvest = m;
+    H->keep_pos = keep_pos;
+    H->keep_neg = keep_neg;
+    return ZK_OK;
+}
+
+zk_status zk_learned_stats(zk_handle *h, int64_t out[6], int reset)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (!H->stats) {
+        H->err = "zk_learned_stats: DG handles only";
+        return ZK_ERR_ARG;
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    unsigned long long v[8] = {0};
+    ZK_HIP(hipMemcpyAsync(v, H->stats, sizeof(v), hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    if (out)
+        for (int k = 0; k < 6; ++k) out[k] = (int64_t)v[k];
+    if (reset) {
+        ZK_HIP(hipMemsetAsync(H->stats, 0, sizeof(v), H->stream));
+        ZK_HIP(hipStreamSynchronize(H->stream));
+    }
+    return ZK_OK;
+}
+
+zk_status zk_mlp_features(zk_handle *h, zk_mlp *m, const int32_t *cells, int n, uint16_t *X, float *scores)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    zk_status st = check_args(H);
+    if (st) return st;
+    if (H->cfg.method != ZK_METHOD_DG || H->cfg.P != 2 || !cells || n <= 0) {
+        H->err = "zk_mlp_features: DG P=2 handle and a non-empty cell list";
+        return ZK_ERR_ARG;
+    }
+    for (int t = 0; t < n; ++t) {
+        const int i = cells[t] / H->g.nx, j = cells[t] % H->g.nx;
+        if (cells[t] < 0 || i < 3 || i >= H->g.ny - 3 || j < 3 || j >= H->g.nx - 3) {
+            H->err = "zk_mlp_features: cells must lie in [3,ny-3)x[3,nx-3)";
+            return ZK_ERR_ARG;
+        }
+    }
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    return zk::mlp_features_of(*H, m, cells, n, X, scores);
 }
 
 zk_status zk_updated_cells(zk_handle *h, int64_t *out)

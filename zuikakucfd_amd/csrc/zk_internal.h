@@ -9,6 +9,8 @@
 
 #include "../../include/zk.h"
 
+struct zk_mlp;
+
 namespace zk {
 
 // Ghost-fill map entry: (source cell index << 2) | negate bits.  Bit 0 negates
@@ -62,6 +64,15 @@ struct Handle {
     unsigned long long *lam_bits = nullptr; // device max-reduce target
     int32_t *bct = nullptr;   // bctype copy (device)
     std::vector<int32_t> h_bct;
+
+    // learned troubled-cell indicator (DG P=2)
+    zk_mlp *mlp = nullptr;           // network used by ZK_LIMITER_LEARNED
+    float mlp_lo = 0.2f, mlp_hi = 0.8f, mlp_audit = 0.02f;
+    zk_mlp *harvest = nullptr;       // dataset receiving (features, TVB flag) pairs
+    float keep_pos = 1.0f, keep_neg = 0.05f;
+    uint32_t stage_counter = 0;
+    unsigned long long *stats = nullptr;  // audited, agree, miss, false flag, mlp flagged, band
+    float *score = nullptr;
 
     // kernel timing: when prof is set every launcher brackets its kernel with
     // a pair of events on the handle's stream

@@ -1,0 +1,119 @@
+"""Learned troubled-cell indicator for DG P=2: harvest, train, evaluate.
+
+The network (zk.Mlp) is trained on (normalised neighbourhood modal
+coefficients, TVB flag) pairs that the GPU solver itself produces while it
+runs the plain TVB limiter on seeded random four-quadrant Riemann problems.
+In use (ZK_LIMITER_LEARNED) scores above `hi` mark a troubled cell, below
+`lo` a smooth one; scores in between and a random audit fraction fall back to
+the TVB test, which also counts the network's misses and false flags.
+"""
+from __future__ import annotations
+
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import meshio, zk
+
+DATA = Path(__file__).resolve().parent / "data"
+WEIGHTS = DATA / "troubled_cell_p2.npz"
+CFL_P2 = 0.9 / (2.0 * (2 * 2 + 1))
+
+
+def dg_solver(m, limiter: int, M: float = 50.0) -> zk.Solver:
+    s = zk.Solver(method=zk.METHOD_DG, P=2, limiter=limiter, tvb_M=M)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    return s
+
+
+def harvest(mlp: zk.Mlp, seeds, n: int = 96, steps: int = 60, keep_pos: float = 1.0,
+            keep_neg: float = 0.05) -> int:
+    """Run plain TVB DG P=2 on each seeded problem, appending pairs to mlp's set."""
+    for seed in seeds:
+        m = meshio.random_riemann(int(seed), n)
+        s = dg_solver(m, zk.LIMITER_TVB)
+        dt = CFL_P2 * m.da / max(s.max_speed(), 1e-7)
+        s.harvest(mlp, keep_pos, keep_neg)
+        s.step(dt, steps)
+        s.sync()
+        s.harvest(None)
+        s.close()
+    return mlp.dataset(0)[0]
+
+
+def train(mlp: zk.Mlp, epochs: int = 30, batch: int = 256, lr: float = 2e-3, seed: int = 0) -> np.ndarray:
+    return mlp.train(epochs, batch, lr, seed)
+
+
+def save(mlp: zk.Mlp, path=WEIGHTS, **meta) -> None:
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    np.savez(path, w=mlp.get_params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path=WEIGHTS, device: int = 0) -> zk.Mlp:
+    d = np.load(path)
+    mlp = zk.Mlp(device=device)
+    mlp.set_params(d["w"])
+    return mlp
+
+
+def evaluate(mlp: zk.Mlp, seed: int, n: int, steps: int, lo: float = 0.2, hi: float = 0.8,
+             audit: float = 0.02) -> dict:
+    """Plain TVB and learned runs of one problem from the same state with the
+    same fixed dt: rates, density L1 difference, the network's miss and
+    false-flag rates on audited cells and the share of cells in the band."""
+    m = meshio.random_riemann(int(seed), n)
+    plain = dg_solver(m, zk.LIMITER_TVB)
+    dt = CFL_P2 * m.da / max(plain.max_speed(), 1e-7)
+    learned = dg_solver(m, zk.LIMITER_LEARNED)
+    learned.attach_mlp(mlp, lo, hi, audit)
+    ncell = plain.updated_cells()
+    out = {"seed": int(seed), "n": n, "steps": steps, "dt": dt, "lo": lo, "hi": hi, "audit": audit}
+    for name, s in (("tvb", plain), ("learned", learned)):
+        s.step(dt, 2)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dt, steps)
+        s.sync()
+        out[f"{name}_cell_stage_per_s"] = 3 * ncell * steps / (time.perf_counter() - t0)
+    Up, Ul = plain.get_conserved(), learned.get_conserved()
+    mask = np.zeros_like(m.bctype, dtype=bool)
+    mask[3:-3, 3:-3] = True
+    out["l1_rho_vs_tvb"] = float(np.abs(Ul[0][mask] - Up[0][mask]).sum() / np.abs(Up[0][mask]).sum())
+    st = learned.learned_stats()
+    a = max(st["audited"], 1)
+    out.update(st)
+    out["miss_rate"] = st["miss"] / a
+    out["false_flag_rate"] = st["false_flag"] / a
+    out["band_fraction"] = st["in_band"] / max(1, 3 * (steps + 2) * ncell)
+    out["finite"] = bool(np.isfinite(Ul).all())
+    plain.close()
+    learned.close()
+    return out
+
+
+def agreement(mlp: zk.Mlp, seed: int, n: int, steps: int) -> dict:
+    """Agreement of the network's 0.5 decision with the TVB test on every
+    cell of a run (audit fraction 1: every decision is the TVB test's)."""
+    m = meshio.random_riemann(int(seed), n)
+    s = dg_solver(m, zk.LIMITER_LEARNED)
+    s.attach_mlp(mlp, 0.0, 1.0, 1.0)
+    dt = CFL_P2 * m.da / max(s.max_speed(), 1e-7)
+    s.step(dt, steps)
+    st = s.learned_stats()
+    s.close()
+    a = max(st["audited"], 1)
+    return {"audited": st["audited"], "agree_rate": st["agree"] / a, "miss_rate": st["miss"] / a,
+            "false_flag_rate": st["false_flag"] / a}
+
+
+def build_default(seeds=range(0, 48), n: int = 96, steps: int = 60, epochs: int = 40, capacity: int = 400000,
+                  path=WEIGHTS) -> dict:
+    """Harvest, train and save the network shipped with the package."""
+    mlp = zk.Mlp(seed=1234, capacity=capacity)
+    count = harvest(mlp, seeds, n=n, steps=steps)
+    losses = train(mlp, epochs=epochs)
+    _, _, y = mlp.dataset(count)
+    save(mlp, path, losses=losses, samples=count, positives=int(y.sum()), seeds=list(seeds), n=n, steps=steps)
+    return {"samples": count, "positives": int(y.sum()), "loss_first": float(losses[0]), "loss_last": float(losses[-1])}

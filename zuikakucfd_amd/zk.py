@@ -32,7 +32,11 @@ EXPORTED = [
     "zk_ensemble_create", "zk_ensemble_add", "zk_ensemble_finalize", "zk_ensemble_set_tuning",
     "zk_ensemble_step", "zk_ensemble_sync", "zk_ensemble_size", "zk_ensemble_get",
     "zk_ensemble_last_error", "zk_ensemble_destroy",
+    "zk_mlp_create", "zk_mlp_destroy", "zk_mlp_last_error", "zk_mlp_param_count", "zk_mlp_get_params",
+    "zk_mlp_set_params", "zk_mlp_grad", "zk_mlp_infer", "zk_mlp_dataset", "zk_mlp_clear_dataset", "zk_mlp_train",
+    "zk_harvest", "zk_attach_mlp", "zk_learned_stats", "zk_mlp_features",
 ]
+_NO_STATUS = ("zk_last_error", "zk_ensemble_last_error", "zk_mlp_last_error", "zk_mlp_param_count")
 
 
 class ZkConfig(C.Structure):
@@ -96,8 +100,27 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_ensemble_last_error.argtypes = [vp]
     lib.zk_ensemble_last_error.restype = C.c_char_p
     lib.zk_ensemble_destroy.argtypes = [vp]
+    u16p = C.POINTER(C.c_uint16)
+    fp = C.POINTER(C.c_float)
+    lib.zk_mlp_create.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(vp)]
+    lib.zk_mlp_destroy.argtypes = [vp]
+    lib.zk_mlp_last_error.argtypes = [vp]
+    lib.zk_mlp_last_error.restype = C.c_char_p
+    lib.zk_mlp_param_count.argtypes = []
+    lib.zk_mlp_param_count.restype = C.c_int
+    lib.zk_mlp_get_params.argtypes = [vp, fp]
+    lib.zk_mlp_set_params.argtypes = [vp, fp]
+    lib.zk_mlp_grad.argtypes = [vp, u16p, fp, C.c_int, fp, fp]
+    lib.zk_mlp_infer.argtypes = [vp, u16p, C.c_int, fp]
+    lib.zk_mlp_dataset.argtypes = [vp, C.POINTER(C.c_int), u16p, fp, C.c_int]
+    lib.zk_mlp_clear_dataset.argtypes = [vp]
+    lib.zk_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, fp]
+    lib.zk_harvest.argtypes = [vp, vp, C.c_float, C.c_float]
+    lib.zk_attach_mlp.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float]
+    lib.zk_learned_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
+    lib.zk_mlp_features.argtypes = [vp, vp, C.POINTER(C.c_int32), C.c_int, u16p, fp]
     for name in EXPORTED:
-        if name not in ("zk_last_error", "zk_ensemble_last_error"):
+        if name not in _NO_STATUS:
             getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib
@@ -214,6 +237,33 @@ class Solver:
     def set_tuning(self, fv_tile: int = 0, dg_variant: int = 0) -> None:
         self._check(self.lib.zk_set_tuning(self.h, int(fv_tile), int(dg_variant)), "zk_set_tuning")
 
+    def attach_mlp(self, mlp: "Mlp", lo: float = 0.2, hi: float = 0.8, audit: float = 0.02) -> None:
+        self._mlp = mlp  # keep the network alive while the handle uses it
+        self._check(self.lib.zk_attach_mlp(self.h, mlp.h, float(lo), float(hi), float(audit)), "zk_attach_mlp")
+
+    def harvest(self, mlp: "Mlp | None", keep_pos: float = 1.0, keep_neg: float = 0.05) -> None:
+        self._harvest = mlp
+        self._check(self.lib.zk_harvest(self.h, mlp.h if mlp is not None else None, float(keep_pos), float(keep_neg)),
+                    "zk_harvest")
+
+    def learned_stats(self, reset: bool = False) -> dict:
+        v = (C.c_int64 * 6)()
+        self._check(self.lib.zk_learned_stats(self.h, v, 1 if reset else 0), "zk_learned_stats")
+        keys = ("audited", "agree", "miss", "false_flag", "net_flagged", "in_band")
+        return dict(zip(keys, list(v)))
+
+    def mlp_features(self, cells, mlp: "Mlp | None" = None) -> tuple[np.ndarray, np.ndarray | None]:
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        n = cells.size
+        X = np.zeros((n, 192), dtype=np.uint16)
+        sc = np.zeros(n, dtype=np.float32)
+        self._check(self.lib.zk_mlp_features(self.h, mlp.h if mlp is not None else None,
+                                             cells.ctypes.data_as(C.POINTER(C.c_int32)), n,
+                                             X.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                             sc.ctypes.data_as(C.POINTER(C.c_float)) if mlp is not None else None),
+                    "zk_mlp_features")
+        return X, (sc if mlp is not None else None)
+
     def profile(self, enable: bool = True) -> None:
         self._check(self.lib.zk_profile(self.h, 1 if enable else 0), "zk_profile")
 
@@ -284,6 +334,88 @@ class Ensemble:
     def close(self) -> None:
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.zk_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u16p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint16))
+
+
+class Mlp:
+    """The troubled-cell network (zk_mlp_* in include/zk.h): FP32 master
+    parameters, BF16 MFMA forward/backward, Adam, and a device-side dataset
+    filled by Solver.harvest."""
+
+    def __init__(self, seed: int = 0, capacity: int = 0, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.zk_mlp_create(int(device), int(seed), int(capacity), C.byref(h))
+        if st != ZK_OK:
+            raise ZkError(f"zk_mlp_create failed ({_ERR_NAMES.get(st, st)})")
+        self.h = h
+        self.nparams = self.lib.zk_mlp_param_count()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != ZK_OK:
+            raise ZkError(f"{what}: {_ERR_NAMES.get(st, st)}: {self.lib.zk_mlp_last_error(self.h).decode()}")
+
+    def get_params(self) -> np.ndarray:
+        w = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self.lib.zk_mlp_get_params(self.h, _f32p(w)), "zk_mlp_get_params")
+        return w
+
+    def set_params(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.nparams:
+            raise ValueError("wrong parameter count")
+        self._check(self.lib.zk_mlp_set_params(self.h, _f32p(w)), "zk_mlp_set_params")
+
+    def grad(self, Xbits: np.ndarray, y: np.ndarray) -> tuple[np.ndarray, float]:
+        Xbits = np.ascontiguousarray(Xbits, dtype=np.uint16)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.zk_mlp_grad(self.h, _u16p(Xbits), _f32p(y), y.size, _f32p(g), C.byref(loss)),
+                    "zk_mlp_grad")
+        return g, loss.value
+
+    def infer(self, Xbits: np.ndarray) -> np.ndarray:
+        Xbits = np.ascontiguousarray(Xbits, dtype=np.uint16)
+        out = np.zeros(Xbits.shape[0], dtype=np.float32)
+        self._check(self.lib.zk_mlp_infer(self.h, _u16p(Xbits), Xbits.shape[0], _f32p(out)), "zk_mlp_infer")
+        return out
+
+    def dataset(self, max_samples: int = 0) -> tuple[int, np.ndarray, np.ndarray]:
+        n = C.c_int()
+        X = np.zeros((max(max_samples, 0), 192), dtype=np.uint16)
+        y = np.zeros(max(max_samples, 0), dtype=np.float32)
+        self._check(self.lib.zk_mlp_dataset(self.h, C.byref(n), _u16p(X) if max_samples else None,
+                                            _f32p(y) if max_samples else None, int(max_samples)), "zk_mlp_dataset")
+        k = min(n.value, max_samples)
+        return n.value, X[:k], y[:k]
+
+    def clear_dataset(self) -> None:
+        self._check(self.lib.zk_mlp_clear_dataset(self.h), "zk_mlp_clear_dataset")
+
+    def train(self, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._check(self.lib.zk_mlp_train(self.h, int(epochs), int(batch), float(lr), int(seed), _f32p(losses)),
+                    "zk_mlp_train")
+        return losses[:epochs]
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.zk_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):
