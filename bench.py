@@ -218,6 +218,33 @@ def path_ensemble(steps: int, ninst: int = 256) -> dict:
             "timing": "wall clock around the launches with a device sync at each end"}
 
 
+def path_learned(steps: int, problems: int, n: int = 2048) -> dict:
+    """Config 4: DG P=2 on seeded random four-quadrant problems at n^2; the
+    learned troubled-cell indicator (shipped network, band [0.2, 0.8], 2%
+    audit) against the plain FP64 TVB limiter from the same state with the
+    same dt: rates, density L1 difference, the network's miss and false-flag
+    rates on audited cells."""
+    from zuikakucfd_amd import learned
+
+    if not learned.WEIGHTS.exists():
+        return {"error": f"no trained network at {learned.WEIGHTS.name}"}
+    mlp = learned.load()
+    rows = [learned.evaluate(mlp, seed, n, steps) for seed in range(problems)]
+    mlp.close()
+    tvb = sum(r["tvb_cell_stage_per_s"] for r in rows) / len(rows)
+    lrn = sum(r["learned_cell_stage_per_s"] for r in rows) / len(rows)
+    tot = {k: sum(r[k] for r in rows) for k in ("audited", "agree", "miss", "false_flag", "net_flagged", "in_band",
+                                                 "candidates", "audited_positive")}
+    return {"config": f"DG P=2 {n}^2, TVB M=50 vs learned indicator (band [0.2, 0.8], 2% audit), "
+                      f"seeds 0..{problems - 1} of 64, {steps} steps each",
+            "steps": steps, "problems": problems, "tvb_cell_stage_per_s": tvb, "learned_cell_stage_per_s": lrn,
+            "speedup": lrn / tvb, "l1_rho_vs_tvb_max": max(r["l1_rho_vs_tvb"] for r in rows),
+            **learned.rates(tot),
+            "candidate_fraction": sum(r["candidate_fraction"] for r in rows) / len(rows),
+            "band_fraction": tot["in_band"] / max(tot["candidates"], 1),
+            "finite": all(r["finite"] for r in rows)}
+
+
 def device_copy(nbytes: int) -> int:
     """One device-to-device hipMemcpy of nbytes (the runtime's plain copy
     kernel), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -245,6 +272,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the other configurations")
     ap.add_argument("--path-steps", type=int, default=20)
+    ap.add_argument("--learned-problems", type=int, default=4,
+                    help="seeded problems of the DG P=2 learned-indicator configuration (of 64)")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB on the device (a plain copy "
@@ -306,7 +335,8 @@ def main() -> int:
     if not args.no_paths:
         line["paths"] = {"weno_fp64_riemann256": path_weno_riemann256(4 * args.path_steps),
                          "jet_fp32_bulk": path_jet(args.path_steps),
-                         "ensemble256": path_ensemble(args.path_steps)}
+                         "ensemble256": path_ensemble(args.path_steps),
+                         "dg2_learned2048": path_learned(args.path_steps, args.learned_problems)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, m, dt)
     if args.calibrate_copy_mib > 0:

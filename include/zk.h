@@ -197,16 +197,21 @@ zk_status zk_mlp_dataset(zk_mlp *m, int *count, uint16_t *X, float *y, int max);
 zk_status zk_mlp_clear_dataset(zk_mlp *m);
 /* Adam over the harvested set: epochs x (count / B) steps of batch B (B % 32 == 0). */
 zk_status zk_mlp_train(zk_mlp *m, int epochs, int B, float lr, uint64_t seed, float *loss_per_epoch);
-/* Harvest (features, TVB flag) pairs during the stages of a DG P=2 TVB handle;
- * positives kept with probability keep_pos, negatives keep_neg; m = NULL stops. */
+/* Harvest (features, TVB flag) pairs during the stages of a DG P=2 TVB handle,
+ * from candidate cells (a linear mode above M h^2; the only cells the learned
+ * limiter scores): positives kept with probability keep_pos, negatives with
+ * keep_neg; samples past m's capacity are dropped; m = NULL stops. */
 zk_status zk_harvest(zk_handle *h, zk_mlp *m, float keep_pos, float keep_neg);
 /* Use m in a ZK_LIMITER_LEARNED handle: scores above hi mark a troubled cell,
  * below lo a smooth one; scores in [lo, hi] and a random `audit` fraction of
  * cells fall back to the TVB test, which also scores the network. */
 zk_status zk_attach_mlp(zk_handle *h, zk_mlp *m, float lo, float hi, float audit);
-/* Counters since the last reset: audited, agreed, missed (TVB troubled, net
- * not), false flags (net troubled, TVB not), net-flagged, in-band. */
-zk_status zk_learned_stats(zk_handle *h, int64_t out[6], int reset);
+/* Counters since the last reset, over candidate cells (those with a linear
+ * mode above M h^2: every other cell passes the TVB test unchanged and is
+ * never scored): audited, agreed, missed (TVB troubled, net not), false
+ * flags (net troubled, TVB not), net-flagged, in-band, candidates, audited
+ * cells the TVB test found troubled. */
+zk_status zk_learned_stats(zk_handle *h, int64_t out[8], int reset);
 /* Features (BF16 bits, [n][192]) and, when m is given, scores of n cells of the
  * current state (cells as row-major indices inside [3,ny-3)x[3,nx-3)). */
 zk_status zk_mlp_features(zk_handle *h, zk_mlp *m, const int32_t *cells, int n, uint16_t *X, float *scores);

@@ -794,28 +794,88 @@ __device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], c
     return any;
 }
 
-// Scores of every updated-range cell (16 per wave, MFMA), written to score[k].
-__global__ __launch_bounds__(256) void dg_score_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src,
-                                                       int nx, int nxu, size_t ncell, size_t np,
-                                                       const __bf16 *__restrict__ w1b, const __bf16 *__restrict__ w2p,
-                                                       const float *__restrict__ w, float *__restrict__ score,
-                                                       const StepScalars *__restrict__ S)
+// A cell can only be changed by the TVB test when one of its linear modes is
+// above M h^2 (the minmod is skipped otherwise); such cells are the
+// candidates the network is asked about.  NaN counts as a candidate, as the
+// test treats it.
+template <int P>
+__device__ __forceinline__ bool tvb_candidate(const double C[4][(P + 1) * (P + 1)], double Mh2)
 {
-    if (S->done) return;
-    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
-    if (t0 >= ncell) return;
-    const int i = threadIdx.x & 15;
-    const size_t t = t0 + i < ncell ? t0 + i : ncell - 1;
-    const size_t k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
-    const float sc = mlp_score16(Q, src, np, nx, k, w1b, w2p, w);
-    if ((threadIdx.x & 63) < 16 && t0 + i < ncell) score[k] = sc;
+    bool c = false;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) c = c || !(fabs(C[v][P + 1]) <= Mh2) || !(fabs(C[v][1]) <= Mh2);
+    return c;
 }
 
-// Learned limiter: outside the uncertainty band [lo, hi] the score decides
-// (troubled = score > hi, then every variable is limited); inside the band and
-// on a random audit fraction the plain TVB test decides.  Audited cells also
-// record agreement of the network's 0.5 decision with the TVB test.  The
-// positivity scaling follows, as in the TVB limiter.
+__global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
+
+// Compacts the candidate cells of the stage into cand[1..cand[0]] (one
+// atomic per wavefront) and counts them in stats[6].
+template <int P>
+__global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__ Q, const int32_t *__restrict__ bct,
+                                                      int nx, int nxu, size_t ncell, size_t np, double Mh2,
+                                                      int32_t *__restrict__ cand,
+                                                      unsigned long long *__restrict__ stats,
+                                                      const StepScalars *__restrict__ S)
+{
+    constexpr int NM = (P + 1) * (P + 1);
+    if (S->done) return;  // uniform over the grid
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool is = false;
+    size_t k = 0;
+    if (t < ncell) {
+        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+        if (bct[k] == 0) {
+            double C[4][NM];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
+                C[c][P + 1] = Q[(size_t)(c * NM + P + 1) * np + k];
+            }
+            is = tvb_candidate<P>(C, Mh2);
+        }
+    }
+    const unsigned long long mask = __ballot(is);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int n = __popcll(mask);
+    int base = 0;
+    if (lane == leader) {
+        base = atomicAdd(cand, n);
+        atomicAdd(stats + 6, (unsigned long long)n);
+    }
+    base = __shfl(base, leader);
+    if (is) cand[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
+}
+
+// Scores the compacted candidates, 16 per wavefront per pass (MFMA), into
+// score[k]; a persistent grid walks the list.
+__global__ __launch_bounds__(256) void dg_score_list_kernel(const double *__restrict__ Q,
+                                                            const int32_t *__restrict__ src, int nx, size_t np,
+                                                            const int32_t *__restrict__ cand,
+                                                            const __bf16 *__restrict__ w1b,
+                                                            const __bf16 *__restrict__ w2p, const float *__restrict__ w,
+                                                            float *__restrict__ score, const StepScalars *__restrict__ S)
+{
+    if (S->done) return;
+    const int n = cand[0];
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    const int i = threadIdx.x & 15;
+    for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16; base < n; base += waves * 16) {
+        const int t = base + i < n ? base + i : n - 1;
+        const size_t k = (size_t)cand[1 + t];
+        const float sc = mlp_score16(Q, src, np, nx, k, w1b, w2p, w);
+        if ((threadIdx.x & 63) < 16 && base + i < n) score[k] = sc;
+    }
+}
+
+// Learned limiter.  Non-candidate cells pass unchanged, as they pass the TVB
+// test.  For candidates, outside the uncertainty band [lo, hi] the score
+// decides (troubled = score > hi, then every variable is limited); inside the
+// band and on a random audit fraction the plain TVB test decides.  Audited
+// cells also record agreement of the network's 0.5 decision with the TVB
+// test.  The positivity scaling follows, as in the TVB limiter.
 template <int P>
 __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q, const double *__restrict__ M,
                                                          const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
@@ -837,9 +897,9 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
 #pragma unroll
         for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
     int flag = 0;
-    if (bct[k] == 0) {
+    if (bct[k] == 0 && tvb_candidate<P>(C, Mh2)) {
         const float sc = score[k];
-        const bool audit = hash3((uint32_t)k, salt, 0x5bd1e995u) < audit_thresh;
+        const bool audit = audit_thresh == 0xFFFFFFFFu || hash3((uint32_t)k, salt, 0x5bd1e995u) < audit_thresh;
         const bool band = sc >= lo && sc <= hi;
         double axl[4], ayl[4];
         bool chg[4];
@@ -848,6 +908,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
             if (audit) {
                 const bool net = sc > 0.5f;
                 atomicAdd(stats + 0, 1ull);
+                if (tv) atomicAdd(stats + 7, 1ull);
                 if (net == tv) atomicAdd(stats + 1, 1ull);
                 else if (tv) atomicAdd(stats + 2, 1ull);
                 else atomicAdd(stats + 3, 1ull);
@@ -885,8 +946,10 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
     }
 }
 
-// Harvest (features, TVB flag) pairs of the combined (pre-limit) stage state.
-// Positives are kept with probability keep_pos, negatives with keep_neg.
+// Harvest (features, TVB flag) pairs of the combined (pre-limit) stage state,
+// from candidate cells only (the cells the network is asked about): positives
+// are kept with probability keep_pos, negatives with keep_neg.  Slots past the
+// capacity are dropped.
 template <int P>
 __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restrict__ Q, const double *__restrict__ M,
                                                          const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
@@ -909,9 +972,12 @@ __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restric
     double axl[4], ayl[4];
     bool chg[4];
     const bool tv = tvb_test<P>(C, M, src, np, k, nx, Mh2, axl, ayl, chg);
-    if (hash3((uint32_t)k, salt, 0x27d4eb2fu) >= (tv ? keep_pos : keep_neg)) return;
+    if (!tvb_candidate<P>(C, Mh2)) return;  // (a TVB positive is always a candidate)
+    const uint32_t keep = tv ? keep_pos : keep_neg;
+    if (keep != 0xFFFFFFFFu && hash3((uint32_t)k, salt, 0x27d4eb2fu) >= keep) return;
+    if (__atomic_load_n(count, __ATOMIC_RELAXED) >= cap) return;  // bounds the counter's overshoot
     const int slot = atomicAdd(count, 1);
-    if (slot >= cap) return;
+    if (slot < 0 || slot >= cap) return;
     double mean[4], scale[4];
     mlp_scales(Q, np, k, mean, scale);
     for (int f = 0; f < 192; ++f) X[(size_t)slot * 192 + f] = (__bf16)mlp_feature(Q, src, np, nx, k, mean, scale, f);
@@ -982,9 +1048,17 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {
+            ProfScope ps(H, "dg_candidates");
+            hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.cand);
+            hipLaunchKernelGGL((dg_cand_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout, H.bct,
+                               H.g.nx, A.nxu, ncell, H.g.np, Mh2, H.cand, H.stats, H.scal);
+        }
+        {
             ProfScope ps(H, "dg_score");
-            hipLaunchKernelGGL(dg_score_kernel, dim3((ncell + 63) / 64), dim3(256), 0, H.stream, Qout, H.src, H.g.nx,
-                               A.nxu, ncell, H.g.np, mlp_w1b(H.mlp), mlp_w2p(H.mlp), mlp_w(H.mlp), H.score, H.scal);
+            const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
+            hipLaunchKernelGGL(dg_score_list_kernel, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
+                               H.stream, Qout, H.src, H.g.nx, H.g.np, H.cand, mlp_w1b(H.mlp), mlp_w2p(H.mlp),
+                               mlp_w(H.mlp), H.score, H.scal);
         }
         ProfScope ps(H, "dg_learned");
         const uint32_t audit = H.mlp_audit >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(H.mlp_audit * 4294967296.0);

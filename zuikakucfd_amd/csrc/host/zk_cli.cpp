@@ -14,6 +14,11 @@
 //   --cfl <nu>               (WENO) auto/time modes use dt = nu*h/lambda instead of the
 //                            reference's dt = lambda*h (src_WENO/solver.cpp:150,169)
 //   --positivity             (WENO) face positivity check with first-order fallback
+//   --weights <file>         (DG P=2, learned limiter) network parameters, raw little-endian
+//                            float32 (zk_mlp_param_count values); default
+//                            <executable dir>/../data/troubled_cell_p2.f32
+//   --band <lo> <hi>         (learned) score band deferred to the TVB test, default 0.2 0.8
+//   --audit <fraction>       (learned) random share of cells also checked by TVB, default 0.02
 //   --dump                   also write result/<step>/state.f64 (all modes, full precision)
 //   --quiet                  no per-step progress lines
 #include <chrono>
@@ -117,6 +122,8 @@ struct Opts {
     bool dump = false;
     bool quiet = false;
     bool positivity = false;
+    std::string weights;
+    float band_lo = 0.2f, band_hi = 0.8f, audit = 0.02f;
 };
 
 bool parse_opts(int argc, char **argv, Opts &o)
@@ -136,6 +143,11 @@ bool parse_opts(int argc, char **argv, Opts &o)
         else if (k == "--M" && val(v)) o.M = std::stod(v);
         else if (k == "--precision" && val(v)) o.precision = v == "fp32" ? ZK_PREC_FP32_BULK : ZK_PREC_FP64;
         else if (k == "--cfl" && val(v)) o.cfl = std::stod(v);
+        else if (k == "--weights" && val(v)) o.weights = v;
+        else if (k == "--band" && a + 2 < argc) {
+            o.band_lo = std::stof(argv[++a]);
+            o.band_hi = std::stof(argv[++a]);
+        } else if (k == "--audit" && val(v)) o.audit = std::stof(v);
         else if (k == "--dump") o.dump = true;
         else if (k == "--quiet") o.quiet = true;
         else if (k == "--positivity") o.positivity = true;
@@ -143,6 +155,33 @@ bool parse_opts(int argc, char **argv, Opts &o)
             std::cerr << "unknown option " << k << "\n";
             return false;
         }
+    }
+    return true;
+}
+
+// Loads the network of the learned indicator and attaches it to h; the
+// network must outlive h's stepping (it is released at exit).
+bool attach_network(zk_handle *h, const Opts &o, zk_mlp **out)
+{
+    std::string path = o.weights;
+    if (path.empty()) {
+        std::error_code ec;
+        const fs::path exe = fs::read_symlink("/proc/self/exe", ec);
+        path = (exe.parent_path().parent_path() / "data" / "troubled_cell_p2.f32").string();
+    }
+    std::vector<float> w(zk_mlp_param_count());
+    std::ifstream f(path, std::ios::binary);
+    if (!f || !f.read(reinterpret_cast<char *>(w.data()), w.size() * sizeof(float))) {
+        std::cerr << "[zk] cannot read " << w.size() << " float32 network parameters from " << path << "\n";
+        return false;
+    }
+    if (zk_mlp_create(0, 0, 0, out) != ZK_OK || zk_mlp_set_params(*out, w.data()) != ZK_OK) {
+        std::cerr << "[zk] network: " << zk_mlp_last_error(*out) << "\n";
+        return false;
+    }
+    if (zk_attach_mlp(h, *out, o.band_lo, o.band_hi, o.audit) != ZK_OK) {
+        std::cerr << "[zk] zk_attach_mlp: " << zk_last_error(h) << "\n";
+        return false;
     }
     return true;
 }
@@ -233,6 +272,12 @@ int main(int argc, char **argv)
     if (zk_load(h, in.ny, in.nx, in.da, in.gamma, in.rho.data(), in.u.data(), in.v.data(), in.p.data(),
                 in.bct.data()) != ZK_OK)
         return fail(h, "zk_load");
+    zk_mlp *net = nullptr;
+    if (cfg.method == ZK_METHOD_DG && o.limiter == ZK_LIMITER_LEARNED && !attach_network(h, o, &net)) {
+        zk_destroy(h);
+        if (net) zk_mlp_destroy(net);
+        return 1;
+    }
 
     double lam = 0.0;
     if (zk_max_speed(h, &lam) != ZK_OK) return fail(h, "zk_max_speed");
@@ -294,6 +339,7 @@ int main(int argc, char **argv)
         std::cerr << "[error] CFL " << cfl_actual << " exceeds the DG P=" << P << " limit " << cfl_limit
                   << "; use dt <= " << safe_dt(lam) << " or 'auto'\n";
         zk_destroy(h);
+        if (net) zk_mlp_destroy(net);
         return 1;
     }
     const int print_interval = std::max(1, time_mode ? 10 : timesteps / 10000);
@@ -352,6 +398,15 @@ int main(int argc, char **argv)
     if (compute_ms > 0)
         std::printf("cell-stage updates/s: %.4e (%lld updated cells x 3 stages x %d steps)\n",
                     3.0 * (double)nupd * step / (compute_ms / 1e3), (long long)nupd, step);
+    if (net) {
+        int64_t st[8];
+        zk_learned_stats(h, st, 0);
+        std::printf("learned indicator: %lld candidate cell-stages, %lld audited, %lld agree, %lld missed, %lld false"
+                    " flags, %lld flagged by the network, %lld deferred (in band)\n", (long long)st[6],
+                    (long long)st[0], (long long)st[1], (long long)st[2], (long long)st[3], (long long)st[4],
+                    (long long)st[5]);
+    }
     zk_destroy(h);
+    if (net) zk_mlp_destroy(net);
     return 0;
 }

@@ -81,14 +81,14 @@ zk_status check_args(Handle *H)
 void free_all(Handle *H)
 {
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
-                    H->scal, H->lam_bits, H->bct, H->stats, H->score};
+                    H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
     H->src = nullptr; H->upd = nullptr; H->ring = nullptr;
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
     H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
-    H->stats = nullptr; H->score = nullptr;
+    H->stats = nullptr; H->score = nullptr; H->cand = nullptr;
     H->loaded = false;
 }
 
@@ -313,6 +313,9 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         if (H->cfg.limiter == ZK_LIMITER_LEARNED) {
             ZK_HIP(hipMalloc(&H->score, np * sizeof(float)));
             ZK_HIP(hipMemset(H->score, 0, np * sizeof(float)));
+            ZK_HIP(hipMalloc(&H->cand, (1 + nxu * nyu) * sizeof(int32_t)));
+            ZK_HIP(hipMemset(H->cand, 0, sizeof(int32_t)));
+            ZK_HIP(hipDeviceGetAttribute(&H->ncu, hipDeviceAttributeMultiprocessorCount, H->cfg.device));
         }
     }
     ZK_HIP(hipMemcpy(H->q, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
@@ -613,7 +616,7 @@ zk_status zk_harvest(zk_handle *h, zk_mlp *m, float keep_pos, float keep_neg)
     return ZK_OK;
 }
 
-zk_status zk_learned_stats(zk_handle *h, int64_t out[6], int reset)
+zk_status zk_learned_stats(zk_handle *h, int64_t out[8], int reset)
 {
     Handle *H = reinterpret_cast<Handle *>(h);
     zk_status st = check_args(H);
@@ -627,7 +630,7 @@ zk_status zk_learned_stats(zk_handle *h, int64_t out[6], int reset)
     ZK_HIP(hipMemcpyAsync(v, H->stats, sizeof(v), hipMemcpyDeviceToHost, H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
     if (out)
-        for (int k = 0; k < 6; ++k) out[k] = (int64_t)v[k];
+        for (int k = 0; k < 8; ++k) out[k] = (int64_t)v[k];
     if (reset) {
         ZK_HIP(hipMemsetAsync(H->stats, 0, sizeof(v), H->stream));
         ZK_HIP(hipStreamSynchronize(H->stream));

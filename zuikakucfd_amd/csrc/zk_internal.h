@@ -71,8 +71,10 @@ struct Handle {
     zk_mlp *harvest = nullptr;       // dataset receiving (features, TVB flag) pairs
     float keep_pos = 1.0f, keep_neg = 0.05f;
     uint32_t stage_counter = 0;
-    unsigned long long *stats = nullptr;  // audited, agree, miss, false flag, mlp flagged, band
+    unsigned long long *stats = nullptr;  // audited, agree, miss, false flag, mlp flagged, band, candidates, audited TVB positives
     float *score = nullptr;
+    int32_t *cand = nullptr;  // [0] = count, then the candidate cells of the stage
+    int ncu = 0;              // compute units of the device
 
     // kernel timing: when prof is set every launcher brackets its kernel with
     // a pair of events on the handle's stream

@@ -3,9 +3,11 @@
 The network (zk.Mlp) is trained on (normalised neighbourhood modal
 coefficients, TVB flag) pairs that the GPU solver itself produces while it
 runs the plain TVB limiter on seeded random four-quadrant Riemann problems.
-In use (ZK_LIMITER_LEARNED) scores above `hi` mark a troubled cell, below
-`lo` a smooth one; scores in between and a random audit fraction fall back to
-the TVB test, which also counts the network's misses and false flags.
+In use (ZK_LIMITER_LEARNED) only candidate cells are scored: those with a
+linear mode above M h^2, since every other cell passes the TVB test
+unchanged.  Scores above `hi` mark a troubled cell, below `lo` a smooth one;
+scores in between and a random audit fraction fall back to the TVB test,
+which also counts the network's misses and false flags.
 """
 from __future__ import annotations
 
@@ -28,8 +30,10 @@ def dg_solver(m, limiter: int, M: float = 50.0) -> zk.Solver:
 
 
 def harvest(mlp: zk.Mlp, seeds, n: int = 96, steps: int = 60, keep_pos: float = 1.0,
-            keep_neg: float = 0.05) -> int:
-    """Run plain TVB DG P=2 on each seeded problem, appending pairs to mlp's set."""
+            keep_neg: float = 1.0) -> int:
+    """Run plain TVB DG P=2 on each seeded problem, appending the (features,
+    flag) pairs of candidate cells to mlp's set (positives kept with
+    probability keep_pos, negatives keep_neg)."""
     for seed in seeds:
         m = meshio.random_riemann(int(seed), n)
         s = dg_solver(m, zk.LIMITER_TVB)
@@ -47,8 +51,13 @@ def train(mlp: zk.Mlp, epochs: int = 30, batch: int = 256, lr: float = 2e-3, see
 
 
 def save(mlp: zk.Mlp, path=WEIGHTS, **meta) -> None:
-    Path(path).parent.mkdir(parents=True, exist_ok=True)
-    np.savez(path, w=mlp.get_params(), **{k: np.asarray(v) for k, v in meta.items()})
+    """Parameters and training metadata as .npz, plus the raw little-endian
+    float32 vector (.f32) that the zk_dg3 executable's --weights reads."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    w = mlp.get_params()
+    np.savez(path, w=w, **{k: np.asarray(v) for k, v in meta.items()})
+    w.astype("<f4").tofile(path.with_suffix(".f32"))
 
 
 def load(path=WEIGHTS, device: int = 0) -> zk.Mlp:
@@ -72,30 +81,41 @@ def evaluate(mlp: zk.Mlp, seed: int, n: int, steps: int, lo: float = 0.2, hi: fl
     out = {"seed": int(seed), "n": n, "steps": steps, "dt": dt, "lo": lo, "hi": hi, "audit": audit}
     for name, s in (("tvb", plain), ("learned", learned)):
         s.step(dt, 2)
-        s.sync()
-        t0 = time.perf_counter()
-        s.step(dt, steps)
-        s.sync()
-        out[f"{name}_cell_stage_per_s"] = 3 * ncell * steps / (time.perf_counter() - t0)
+        ms = s.step_timed(dt, steps)  # HIP events on the handle's stream
+        out[f"{name}_cell_stage_per_s"] = 3 * ncell * steps / (ms / 1e3)
     Up, Ul = plain.get_conserved(), learned.get_conserved()
     mask = np.zeros_like(m.bctype, dtype=bool)
     mask[3:-3, 3:-3] = True
     out["l1_rho_vs_tvb"] = float(np.abs(Ul[0][mask] - Up[0][mask]).sum() / np.abs(Up[0][mask]).sum())
     st = learned.learned_stats()
-    a = max(st["audited"], 1)
     out.update(st)
-    out["miss_rate"] = st["miss"] / a
-    out["false_flag_rate"] = st["false_flag"] / a
-    out["band_fraction"] = st["in_band"] / max(1, 3 * (steps + 2) * ncell)
+    out.update(rates(st))
+    out["candidate_fraction"] = st["candidates"] / max(1, 3 * (steps + 2) * ncell)
+    out["band_fraction"] = st["in_band"] / max(1, st["candidates"])
     out["finite"] = bool(np.isfinite(Ul).all())
     plain.close()
     learned.close()
     return out
 
 
+def rates(st: dict) -> dict:
+    """Classification rates of the network's 0.5 decision against the TVB
+    test over the audited candidate cells: agreement, recall of troubled
+    cells, specificity on smooth ones and their mean (balanced accuracy,
+    which a constant answer scores 0.5 on, whatever the class balance)."""
+    a = max(st["audited"], 1)
+    pos = st["audited_positive"]
+    neg = st["audited"] - pos
+    recall = (pos - st["miss"]) / pos if pos else 1.0
+    spec = (neg - st["false_flag"]) / neg if neg else 1.0
+    return {"agree_rate": st["agree"] / a, "miss_rate": st["miss"] / a, "false_flag_rate": st["false_flag"] / a,
+            "recall": recall, "specificity": spec, "balanced_accuracy": 0.5 * (recall + spec),
+            "positive_share": pos / a}
+
+
 def agreement(mlp: zk.Mlp, seed: int, n: int, steps: int) -> dict:
-    """Agreement of the network's 0.5 decision with the TVB test on every
-    cell of a run (audit fraction 1: every decision is the TVB test's)."""
+    """The network's decisions against the TVB test on every candidate cell
+    of a run (audit fraction 1: every limiting decision is the TVB test's)."""
     m = meshio.random_riemann(int(seed), n)
     s = dg_solver(m, zk.LIMITER_LEARNED)
     s.attach_mlp(mlp, 0.0, 1.0, 1.0)
@@ -103,9 +123,7 @@ def agreement(mlp: zk.Mlp, seed: int, n: int, steps: int) -> dict:
     s.step(dt, steps)
     st = s.learned_stats()
     s.close()
-    a = max(st["audited"], 1)
-    return {"audited": st["audited"], "agree_rate": st["agree"] / a, "miss_rate": st["miss"] / a,
-            "false_flag_rate": st["false_flag"] / a}
+    return {"audited": st["audited"], **rates(st)}
 
 
 def build_default(seeds=range(0, 48), n: int = 96, steps: int = 60, epochs: int = 40, capacity: int = 400000,
@@ -117,3 +135,41 @@ def build_default(seeds=range(0, 48), n: int = 96, steps: int = 60, epochs: int 
     _, _, y = mlp.dataset(count)
     save(mlp, path, losses=losses, samples=count, positives=int(y.sum()), seeds=list(seeds), n=n, steps=steps)
     return {"samples": count, "positives": int(y.sum()), "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="harvest, train and save the DG P=2 troubled-cell network")
+    ap.add_argument("--out", default=str(WEIGHTS))
+    ap.add_argument("--n", type=int, default=2048, help="grid size of the harvest problems")
+    ap.add_argument("--seeds", type=int, nargs=2, default=(100, 112), help="harvest seeds [a, b)")
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--keep-pos", type=float, default=0.03, help="share of troubled candidate cells kept")
+    ap.add_argument("--keep-neg", type=float, default=0.5, help="share of smooth candidate cells kept")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--capacity", type=int, default=1_500_000)
+    ap.add_argument("--eval-seeds", type=int, nargs="*", default=[0, 1])
+    ap.add_argument("--eval-steps", type=int, default=20)
+    a = ap.parse_args(argv)
+    mlp = zk.Mlp(seed=1234, capacity=a.capacity)
+    t0 = time.perf_counter()
+    for seed in range(*a.seeds):
+        count = harvest(mlp, [seed], n=a.n, steps=a.steps, keep_pos=a.keep_pos, keep_neg=a.keep_neg)
+        print(f"[harvest] seed {seed}: {count} samples ({time.perf_counter() - t0:.1f} s)", flush=True)
+    _, _, y = mlp.dataset(count)
+    losses = train(mlp, epochs=a.epochs)
+    print(f"[train] {count} samples, {int(y.sum())} positive; loss {losses[0]:.4f} -> {losses[-1]:.4f} "
+          f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    save(mlp, a.out, losses=losses, samples=count, positives=int(y.sum()), seeds=list(range(*a.seeds)), n=a.n,
+         steps=a.steps, keep_pos=a.keep_pos, keep_neg=a.keep_neg)
+    for seed in a.eval_seeds:
+        print(json.dumps({"agreement": agreement(mlp, seed, a.n, a.eval_steps)}), flush=True)
+        print(json.dumps({"evaluate": evaluate(mlp, seed, a.n, a.eval_steps)}), flush=True)
+    mlp.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

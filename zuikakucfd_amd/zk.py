@@ -247,9 +247,9 @@ class Solver:
                     "zk_harvest")
 
     def learned_stats(self, reset: bool = False) -> dict:
-        v = (C.c_int64 * 6)()
+        v = (C.c_int64 * 8)()
         self._check(self.lib.zk_learned_stats(self.h, v, 1 if reset else 0), "zk_learned_stats")
-        keys = ("audited", "agree", "miss", "false_flag", "net_flagged", "in_band")
+        keys = ("audited", "agree", "miss", "false_flag", "net_flagged", "in_band", "candidates", "audited_positive")
         return dict(zip(keys, list(v)))
 
     def mlp_features(self, cells, mlp: "Mlp | None" = None) -> tuple[np.ndarray, np.ndarray | None]:
