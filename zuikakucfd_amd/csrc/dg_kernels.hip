@@ -141,8 +141,8 @@ DgTables make_tables(int P)
 __device__ __forceinline__ void euler_fg(const double U[4], double gam, double F[4], double G[4])
 {
     constexpr double eps = 1e-3;
-    const double rho = smax(U[0], eps);
-    const double u = U[1] / rho, v = U[2] / rho;
+    const double rho = smax(U[0], eps), ir = 1.0 / rho;
+    const double u = U[1] * ir, v = U[2] * ir;
     const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
     F[0] = rho * u; F[1] = rho * u * u + p; F[2] = rho * u * v; F[3] = (U[3] + p) * u;
     G[0] = rho * v; G[1] = rho * u * v; G[2] = rho * v * v + p; G[3] = (U[3] + p) * v;
@@ -261,12 +261,12 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
         double Fs[4];
         if (X) {
             if (A.flux == ZK_FLUX_HLL) hll_f64(UL, UR, A.gamma, Fs);
-            else hllc_dg_f64(UL, UR, A.gamma, Fs);
+            else hllc_dg_fast_f64(UL, UR, A.gamma, Fs);
         } else {
             const double a[4] = {UL[0], UL[2], UL[1], UL[3]}, b[4] = {UR[0], UR[2], UR[1], UR[3]};
             double Gs[4];
             if (A.flux == ZK_FLUX_HLL) hll_f64(a, b, A.gamma, Gs);
-            else hllc_dg_f64(a, b, A.gamma, Gs);
+            else hllc_dg_fast_f64(a, b, A.gamma, Gs);
             Fs[0] = Gs[0]; Fs[1] = Gs[2]; Fs[2] = Gs[1]; Fs[3] = Gs[3];
         }
 #pragma unroll
@@ -604,25 +604,26 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
     block_max_to(mx, out);
 }
 
+// Cells outside the update range: the stage combination with dU = 0, one
+// thread per (plane, ring cell) so that the launch fills the device.
 __global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__restrict__ Qin,
                                double *__restrict__ Qout, const int32_t *__restrict__ src,
                                const int32_t *__restrict__ ring, int nring, size_t np, int nm, double ca,
                                double cb, int stage, const StepScalars *__restrict__ S, double *__restrict__ M)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nring || S->done) return;
-    const size_t k = (size_t)ring[t];
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nring * 4 * nm || S->done) return;
+    const int p = (int)(t / nring);
+    const size_t k = (size_t)ring[t % nring];
     const int32_t e = src[k];
     const size_t s = (size_t)(e >> kSrcShift);
-    for (int p = 0; p < 4 * nm; ++p) {
-        const int var = p / nm;
-        const double qn = Qn[p * np + k];
-        double qf = Qin[p * np + s];
-        if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
-        const double out = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
-        Qout[p * np + k] = out;
-        if (M && p % nm == 0) M[(size_t)var * np + k] = out;
-    }
+    const int var = p / nm;
+    const double qn = Qn[p * np + k];
+    double qf = Qin[p * np + s];
+    if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
+    const double out = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
+    Qout[p * np + k] = out;
+    if (M && p % nm == 0) M[(size_t)var * np + k] = out;
 }
 
 
@@ -753,6 +754,428 @@ __global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage kernel (P = 1..3; TVB, learned or no limiter): faces, volume
+// term and Shu-Osher combination in one pass over the state.
+//
+// A wave owns a run of 16 adjacent columns (lanes col = 0..15; columns 1..14
+// are updated, 0 and 15 supply the outer face traces) and marches down a
+// segment of R rows, plus one halo row above and one below.  Per row:
+//   * the modes of the 16 cells are loaded once (lane (g, col) holds modes
+//     px = v, py = g, v = 0..3, in 4x4 slots; P < 3 leaves slots past P zero);
+//   * one FP64 MFMA product per variable evaluates all 16 face-trace points of
+//     every cell (rows r = q + 4 side, sides W, E, N, S; lane group g receives
+//     quadrature point q = g of each side);
+//   * the volume integral is the MFMA evaluation / projection of dg_cell_mfma3
+//     generalised to P = 1..3 (quadrature slots past P + 1 have zero weight);
+//   * x faces pair lane col's E trace with lane col+1's W trace (a shuffle in
+//     the 16-lane row); y faces pair the carried S trace of the row above with
+//     this row's N trace, so every face flux is computed once;
+//   * the flux projections onto the face modes are MFMA products whose
+//     A operand repeats the weights so that every lane receives exactly the
+//     coefficients its own modes need.
+// Row r is finished (combined and stored) when row r+1 supplies its S face.
+// ---------------------------------------------------------------------------
+struct MfmaTabsF {
+    double phiA[4][64];  // evaluation at volume point p = 4a + b of mode (px = s, py = l/16)
+    double wfA[4][64];   // projection of F at point (a = s, b = l/16) onto mode l%16 = 4px + py
+    double wgA[4][64];   // same for G
+    double trA[4][64];   // evaluation at trace point r = l%16 (q = r%4, side = r/4) of mode (s, l/16)
+    double axA[64];      // x-face projection: row i -> w_q L_{i%4}(s_q), column q = l/16
+    double ayA[64];      // y-face projection: row i -> w_q L_{i/4}(s_q)
+    double zsA[4][64];   // positivity points: truncated basis() at (a, b), point l%16 = 4a + b
+};
+__constant__ MfmaTabsF c_mff[3];
+
+MfmaTabsF make_mfma_tables_fused(int P)
+{
+    MfmaTabsF M{};
+    const int NQ = P + 1;
+    const double *gp = GLP_H[NQ], *gw = GLW_H[NQ];
+    // quadrature slot -> location and weight; slots past NQ sit at 0 with weight 0
+    auto loc = [&](int q) { return q < NQ ? gp[q] : 0.0; };
+    auto wt = [&](int q) { return q < NQ ? gw[q] : 0.0; };
+    auto L = [&](int n, double x) { return n <= P ? legP_h(n, x) : 0.0; };
+    auto D = [&](int n, double x) { return n <= P ? legD_h(n, x) : 0.0; };
+    for (int s = 0; s < 4; ++s)
+        for (int l = 0; l < 64; ++l) {
+            const int i = l % 16, k = l / 16;
+            {
+                const int a = i / 4, b = i % 4, px = s, py = k;
+                M.phiA[s][l] = L(px, loc(a)) * L(py, loc(b));
+            }
+            {
+                const int px = i / 4, py = i % 4, a = s, b = k;
+                const double ww = wt(a) * wt(b);
+                M.wfA[s][l] = ww * D(px, loc(a)) * L(py, loc(b));
+                M.wgA[s][l] = ww * L(px, loc(a)) * D(py, loc(b));
+            }
+            {
+                const int q = i % 4, side = i / 4, px = s, py = k;
+                double v = 0.0;
+                switch (side) {
+                case 0: v = L(px, -1.0) * L(py, loc(q)); break;  // W, xi = -1
+                case 1: v = L(px, +1.0) * L(py, loc(q)); break;  // E, xi = +1
+                case 2: v = L(px, loc(q)) * L(py, -1.0); break;  // N, eta = -1 (row i-1 side)
+                default: v = L(px, loc(q)) * L(py, +1.0); break; // S, eta = +1 (row i+1 side)
+                }
+                M.trA[s][l] = v;
+            }
+        }
+    for (int l = 0; l < 64; ++l) {
+        const int i = l % 16, q = l / 16;
+        M.axA[l] = wt(q) * L(i % 4, loc(q));
+        M.ayA[l] = wt(q) * L(i / 4, loc(q));
+    }
+    // the 4x4 positivity grid of every degree uses the 4-point Gauss nodes and
+    // the reference's basis(), which drops degree 3 (src_DG/fluid.cpp:897-991)
+    const double zsp[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+    for (int s = 0; s < 4; ++s)
+        for (int l = 0; l < 64; ++l) {
+            const int i = l % 16, a = i / 4, b = i % 4, px = s, py = l / 16;
+            M.zsA[s][l] = (px <= P && py <= P) ? zsP_h(px, zsp[a]) * zsP_h(py, zsp[b]) : 0.0;
+        }
+    return M;
+}
+
+template <int P, int MINW>
+__global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const MfmaTabsF &M = c_mff[P - 1];
+    const DgTables &T = c_tab[P - 1];
+    if (A.S->done) return;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, col = lane & 15;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nseg = (A.nyu + R - 1) / R;
+    if (w >= nrun * nseg) return;  // whole wave
+    const int run = w % nrun, seg = w / nrun;
+    const int j = 2 + run * 14 + col;                 // column of this lane
+    const int jc = j < A.nx ? j : A.nx - 1;           // clamped for loads past the right edge
+    const bool store_col = col >= 1 && col <= 14 && j <= A.nx - 4;
+    const int r0 = 3 + seg * R;
+    const int r1 = (r0 + R < A.ny - 3) ? r0 + R : A.ny - 3;  // rows [r0, r1), halo rows r0-1 and r1
+    const size_t np = A.np;
+    const bool slot = g <= P;                         // this lane group holds real modes
+    const double gam = A.gamma, dt = A.S->dt, inv2h = 0.5 / A.da;
+    // per-slot factor of dU in the combination: dt (stage 1) or cb*dt, times invMass * 0.5/h
+    double fdu[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int m = v * NQ + g;
+        const double sc = (v <= P && slot) ? T.invm[m < NM ? m : 0] * inv2h : 0.0;
+        fdu[v] = sc;
+    }
+
+    // Addressing: plane (c, px = v) is a wave-uniform pointer; a lane adds the
+    // 32-bit element offset g*np + cell, so addresses need no 64-bit VGPR pairs.
+    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    auto load = [&](int row, double Q[4][4]) {
+        const size_t k = (size_t)row * A.nx + jc;
+        const int32_t e = A.src[k];
+        const uint32_t o = gofs + (uint32_t)(e >> kSrcShift);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                Q[c][v] = (v <= P && slot) ? (A.Qin + (size_t)(c * NM + v * NQ) * np)[o] : 0.0;
+        if (e & 1)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Q[1][v] = -Q[1][v];
+        if (e & 2)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Q[2][v] = -Q[2][v];
+    };
+
+    // S trace of the halo row above the segment
+    double Sup[4];
+    {
+        double Q[4][4];
+        load(r0 - 1, Q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s <= P; ++s) t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][lane], Q[c][s], t, 0, 0, 0);
+            Sup[c] = t[3];
+        }
+    }
+    // pending row r-1: out = base - fs * gS, its S-face projection gS still to come
+    double base[4][4];
+    double fs[4];
+    size_t kp = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) base[c][v] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) fs[v] = 0.0;
+
+    for (int r = r0; r <= r1; ++r) {
+        // the table operands are re-read from constant memory every row: an
+        // opaque copy of the lane index keeps the compiler from holding all of
+        // them in registers across the loop
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const bool inner = r < r1;
+        const size_t k = (size_t)r * A.nx + jc;
+        double Q[4][4];
+        load(r, Q);
+        // face traces of this row: element side (W, E, N, S) at quadrature point g
+        d4 Tr[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s <= P; ++s) t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][ln], Q[c][s], t, 0, 0, 0);
+            Tr[c] = t;
+        }
+        // y face between row r-1 (S trace Sup) and row r (N trace), projected onto px = v
+        d4 gN[4];
+        {
+            const double a[4] = {Sup[0], Sup[2], Sup[1], Sup[3]};
+            const double b[4] = {Tr[0][2], Tr[2][2], Tr[1][2], Tr[3][2]};
+            double Gs[4];
+            if (A.flux == ZK_FLUX_HLL) hll_f64(a, b, gam, Gs);
+            else hllc_dg_fast_f64(a, b, gam, Gs);
+            const double G[4] = {Gs[0], Gs[2], Gs[1], Gs[3]};
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                gN[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(M.ayA[ln], G[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+        }
+        if (r > r0 && store_col && slot) {
+            // finish row r-1: its S face is this row's N face
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v <= P; ++v) {
+                    const double out = base[c][v] - fs[v] * gN[c][v];
+                    (A.Qout + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)kp] = out;
+                    if (A.means && v == 0 && g == 0) A.means[(size_t)c * np + kp] = out;
+                }
+        }
+        if (!inner) break;
+        // U^n of row r, needed at the end of this iteration
+        double qn[4][4];
+        if (store_col && slot)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v <= P; ++v) qn[c][v] = (A.Qn + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+        // rhs of row r, all terms but its S face: N face, x faces, then the volume
+        // projection accumulated on top by the MFMA
+        d4 acc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double sN = T.sgn[g < NQ ? g : 0];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[c][v] = sN * gN[c][v];
+        }
+        {
+            double Lx[4], Rx[4], Fx[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                Lx[c] = Tr[c][1];
+                Rx[c] = __shfl_down(Tr[c][0], 1, 16);
+                Sup[c] = Tr[c][3];
+            }
+            if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
+            else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const d4 ge = __builtin_amdgcn_mfma_f64_16x16x4f64(M.axA[ln], Fx[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                const double gE = ge[0];                   // py = g projection of face (col | col+1)
+                const double gW = __shfl_up(gE, 1, 16);    // face (col-1 | col)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[c][v] += T.sgn[v < NQ ? v : 0] * gW - gE;
+            }
+        }
+        {
+            d4 U[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s <= P; ++s) t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.phiA[s][ln], Q[c][s], t, 0, 0, 0);
+                U[c] = t;
+            }
+#pragma unroll
+            for (int v = 0; v <= P; ++v) {
+                const double Up[4] = {U[0][v], U[1][v], U[2][v], U[3][v]};
+                double F[4], G[4];
+                euler_fg(Up, gam, F, G);
+                const double wf = M.wfA[v][ln], wg = M.wgA[v][ln];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(wf, F[c], acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(wg, G[c], acc[c], 0, 0, 0);
+                }
+            }
+        }
+        // combination of row r up to its S-face term
+        const bool upd = A.upd[k] != 0;
+        const double cdt = (A.stage == 0) ? dt : A.cb * dt;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
+        if (store_col && slot) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v <= P; ++v) {
+                    const double du = upd ? acc[c][v] * fdu[v] : 0.0;
+                    base[c][v] = (A.stage == 0) ? qn[c][v] + dt * du : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
+                }
+        }
+        kp = k;
+    }
+}
+
+// TVB limiter then positivity scaling in the matrix-core layout of the fused
+// kernel: a wave owns 16 consecutive cells of the update range, lane (g, col)
+// holds modes (px = v, py = g) of cell col.  The TVB test of a cell runs in its
+// lane of group 0 (mean and x-linear mode there, the y-linear mode shuffled in
+// from group 1) and its outcome is broadcast down the column; the positivity
+// scaling evaluates rho and then all variables at the 4x4 points with MFMA
+// products and takes the minimum theta across the column's lanes.  Same
+// results as dg_tvb_kernel up to the summation order of the point values.
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+                                                          const int32_t *__restrict__ src,
+                                                          const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
+                                                          int ny, int nx, size_t np, double Mh2, double gamma,
+                                                          const StepScalars *__restrict__ S)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const MfmaTabsF &MT = c_mff[P - 1];
+    if (S->done) return;
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t ncell = (size_t)nxu * nyu;
+    const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (t0 >= ncell) return;  // whole wave
+    const bool live = t0 + col < ncell;
+    const size_t t = live ? t0 + col : ncell - 1;
+    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+    const bool slot = g <= P;
+    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    double C[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C[c][v] = (v <= P && slot) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+
+    // ---- TVB (cells with bctype 0) ----
+    int chg = 0;  // bit c: variable c limited
+    double axl[4], ayl[4];
+    const bool tvb_cell = bct[k] == 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double ay_in = __shfl(C[c][0], 16 + col);   // mode (0, 1) of this column's cell
+        const double ub = C[c][0], ax = C[c][1];          // in group 0: modes (0, 0) and (1, 0)
+        double a_x = ax, a_y = ay_in;
+        if (g == 0 && tvb_cell) {
+            if (!(fabs(ax) <= Mh2))
+                a_x = minmod3(ax, filled_mean(Mn, src, np, k + 1, c) - ub, ub - filled_mean(Mn, src, np, k - 1, c));
+            if (!(fabs(ay_in) <= Mh2))
+                a_y = minmod3(ay_in, filled_mean(Mn, src, np, k + nx, c) - ub, ub - filled_mean(Mn, src, np, k - nx, c));
+            if (a_x != ax || a_y != ay_in) chg |= 1 << c;
+        }
+        axl[c] = __shfl(a_x, col);
+        ayl[c] = __shfl(a_y, col);
+    }
+    chg = __shfl(chg, col);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (chg & (1 << c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double nv = 0.0;
+                if (v == 0 && g == 0) nv = C[c][0];
+                else if (v == 1 && g == 0) nv = axl[c];
+                else if (v == 0 && g == 1) nv = ayl[c];
+                C[c][v] = nv;
+            }
+
+    // ---- positivity scaling (every cell of the range), as zs_cell ----
+    constexpr double RHO_MIN = 1e-5, P_MIN = 1e-5, EPS = 1e-5;
+    bool changed = false;
+    double m0[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m0[c] = __shfl(C[c][0], col);
+    if (m0[0] < RHO_MIN) changed = true;
+    m0[0] = smax(m0[0], RHO_MIN);
+    {
+        const double p0 = (gamma - 1.0) * (m0[3] - 0.5 * (m0[1] * m0[1] + m0[2] * m0[2]) / m0[0]);
+        if (p0 < P_MIN) {
+            m0[3] = P_MIN / (gamma - 1.0) + 0.5 * (m0[1] * m0[1] + m0[2] * m0[2]) / m0[0];
+            changed = true;
+        }
+    }
+    if (g == 0) {
+        C[0][0] = m0[0];
+        C[3][0] = m0[3];
+    }
+    // theta_rho from rho at the 16 points (lane holds points a = v, b = g)
+    {
+        d4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s <= P; ++s) z = __builtin_amdgcn_mfma_f64_16x16x4f64(MT.zsA[s][lane], C[0][s], z, 0, 0, 0);
+        double th = 1.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (z[v] < RHO_MIN) th = smin(th, (m0[0] - RHO_MIN) / (m0[0] - z[v] + EPS));
+        th = smin(th, __shfl_xor(th, 16));
+        th = smin(th, __shfl_xor(th, 32));
+        if (th < 1.0) {
+            changed = true;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (v != 0 || g != 0) C[c][v] *= th;
+        }
+    }
+    // theta_p from all variables at the 16 points
+    {
+        d4 z[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            d4 a = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s <= P; ++s) a = __builtin_amdgcn_mfma_f64_16x16x4f64(MT.zsA[s][lane], C[c][s], a, 0, 0, 0);
+            z[c] = a;
+        }
+        const double p_avg = (gamma - 1.0) * (m0[3] - 0.5 * (m0[1] * m0[1] + m0[2] * m0[2]) / m0[0]);
+        double th = 1.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double rho = z[0][v];
+            if (rho <= RHO_MIN) continue;
+            const double u = z[1][v] / rho, w = z[2][v] / rho;
+            const double p = (gamma - 1.0) * (z[3][v] - 0.5 * rho * (u * u + w * w));
+            if (p < P_MIN) th = smin(th, (p_avg - P_MIN) / (p_avg - p + EPS));
+        }
+        th = smin(th, __shfl_xor(th, 16));
+        th = smin(th, __shfl_xor(th, 32));
+        if (th < 1.0) {
+            changed = true;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (v != 0 || g != 0) C[c][v] *= th;
+        }
+    }
+    if (!live) return;
+    if (g == 0) flags[k] = (uint8_t)(chg != 0);
+    if ((chg != 0 || changed) && slot)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] = C[c][v];
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,7 +1428,28 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.flux = H.cfg.flux;
     A.limiter = H.cfg.limiter;
     const size_t ncell = (size_t)A.nxu * A.nyu;
-    if (ncell > 0) {
+    // variant: 0 auto (fused), -1 split with the VALU cell kernel, 2..4 split with the
+    // P=3 MFMA cell kernel at that occupancy, 10+R fused with R-row segments
+    // auto picks the fused kernel for P = 3; at P <= 2 its padded 4x4 slots waste
+    // more than the split kernels' extra traffic costs (measured on 1024^2)
+    const bool fused = H.cfg.limiter != ZK_LIMITER_PERSSON_ZS && !H.no_mfma &&
+                       ((H.tune == 0 && P == 3) || H.tune >= 10);
+    if (ncell > 0 && fused) {
+        int R = H.tune >= 10 ? H.tune - 10 : 0;
+        const int nrun = (A.nxu + 13) / 14;
+        if (R <= 0) {
+            // about four segment waves per SIMD over the device, rows in [8, 64]
+            const int simds = 4 * (H.ncu > 0 ? H.ncu : 256);
+            const long target = 4L * simds;
+            const long nseg = (target + nrun - 1) / nrun;
+            R = (int)((A.nyu + nseg - 1) / nseg);
+            R = R < 8 ? 8 : (R > 64 ? 64 : R);
+        }
+        const long waves = (long)nrun * ((A.nyu + R - 1) / R);
+        ProfScope ps(H, "dg_fused");
+        hipLaunchKernelGGL((dg_fused_kernel<P, 2>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, H.stream, A, R,
+                           nrun);
+    } else if (ncell > 0) {
         {
             ProfScope ps(H, "dg_face_x");
             hipLaunchKernelGGL((dg_face_kernel<P, true>), dim3((A.nfx + 127) / 128), dim3(128), 0, H.stream, A);
@@ -1028,7 +1472,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     }
     if (H.nring > 0) {
         ProfScope ps(H, "dg_ring");
-        hipLaunchKernelGGL(dg_ring_kernel, dim3((H.nring + 255) / 256), dim3(256), 0, H.stream, Qn, Qin, Qout,
+        const size_t nt = (size_t)H.nring * 4 * H.nm;
+        hipLaunchKernelGGL(dg_ring_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, H.stream, Qn, Qin, Qout,
                            H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means);
     }
     const double Mh2 = H.cfg.tvb_M * H.g.da * H.g.da;
@@ -1043,8 +1488,12 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB) {
         ProfScope ps(H, "dg_tvb");
-        hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                           H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
+        if (H.no_mfma || P < 3)  // the per-thread kernel is faster at P <= 2 (1024^2)
+            hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
+                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
+        else
+            hipLaunchKernelGGL((dg_tvb_mfma_kernel<P>), dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, H.stream,
+                               Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {
@@ -1077,6 +1526,8 @@ void dg_upload_tables(int device)
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), T, sizeof(T), 0, hipMemcpyHostToDevice);
     const MfmaTabs3 M = make_mfma_tables3();
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_mf3), &M, sizeof(M), 0, hipMemcpyHostToDevice);
+    const MfmaTabsF F[3] = {make_mfma_tables_fused(1), make_mfma_tables_fused(2), make_mfma_tables_fused(3)};
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_mff), F, sizeof(F), 0, hipMemcpyHostToDevice);
     if (device >= 0 && device < 64) g_tables_ready[device] = true;
 }
 

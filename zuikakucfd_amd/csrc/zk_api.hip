@@ -315,8 +315,8 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
             ZK_HIP(hipMemset(H->score, 0, np * sizeof(float)));
             ZK_HIP(hipMalloc(&H->cand, (1 + nxu * nyu) * sizeof(int32_t)));
             ZK_HIP(hipMemset(H->cand, 0, sizeof(int32_t)));
-            ZK_HIP(hipDeviceGetAttribute(&H->ncu, hipDeviceAttributeMultiprocessorCount, H->cfg.device));
         }
+        ZK_HIP(hipDeviceGetAttribute(&H->ncu, hipDeviceAttributeMultiprocessorCount, H->cfg.device));
     }
     ZK_HIP(hipMemcpy(H->q, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
     ZK_HIP(hipMemcpy(H->qa, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));

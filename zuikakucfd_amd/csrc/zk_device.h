@@ -92,22 +92,28 @@ __device__ __forceinline__ void hllc_fv_f64(const double L[4], const double R[4]
     F[3] = FF3 + S * (s3 - U3);
 }
 
-// HLLC of the DG solver (hllcFlux, src_DG/fluid.cpp:450-520).
-__device__ __forceinline__ void hllc_dg_f64(const double L[4], const double R[4], double gamma, double F[4])
+// HLLC of the DG solver (hllcFlux, src_DG/fluid.cpp:450-520): eps 1e-5,
+// sound speed sqrt(max(gamma p / rho, eps)), Roe speeds with u only, the
+// denominator guards and the non-finite star-term reset.  The divisions by
+// rho_L, rho_R and the Roe denominator are one reciprocal each (6 divisions
+// instead of 15), which moves results by a few ulps; the special cases
+// behave identically because the two divisions they involve are kept.
+__device__ __forceinline__ void hllc_dg_fast_f64(const double L[4], const double R[4], double gamma, double F[4])
 {
     constexpr double eps = 1e-5;
-    const double rhoL = smax(L[0], eps), uL = L[1] / rhoL, vL = L[2] / rhoL, EL = L[3] / rhoL;
+    const double rhoL = smax(L[0], eps), iL = 1.0 / rhoL;
+    const double uL = L[1] * iL, vL = L[2] * iL, EL = L[3] * iL;
     const double pL = smax((gamma - 1.0) * (L[3] - 0.5 * rhoL * (uL * uL + vL * vL)), eps);
-    const double aL = sqrt(smax(gamma * pL / rhoL, eps));
-    const double rhoR = smax(R[0], eps), uR = R[1] / rhoR, vR = R[2] / rhoR, ER = R[3] / rhoR;
+    const double aL = sqrt(smax(gamma * pL * iL, eps));
+    const double rhoR = smax(R[0], eps), iR = 1.0 / rhoR;
+    const double uR = R[1] * iR, vR = R[2] * iR, ER = R[3] * iR;
     const double pR = smax((gamma - 1.0) * (R[3] - 0.5 * rhoR * (uR * uR + vR * vR)), eps);
-    const double aR = sqrt(smax(gamma * pR / rhoR, eps));
+    const double aR = sqrt(smax(gamma * pR * iR, eps));
 
     const double sqL = sqrt(rhoL), sqR = sqrt(rhoR);
-    double den = sqL + sqR;
-    if (den < eps) den = eps;
-    const double u_roe = (sqL * uL + sqR * uR) / den;
-    const double h_roe = (sqL * (EL + pL / rhoL) + sqR * (ER + pR / rhoR)) / den;
+    const double iden = 1.0 / smax(sqL + sqR, eps);
+    const double u_roe = (sqL * uL + sqR * uR) * iden;
+    const double h_roe = (sqL * (EL + pL * iL) + sqR * (ER + pR * iR)) * iden;
     const double c_roe = sqrt(smax((gamma - 1.0) * (h_roe - 0.5 * u_roe * u_roe), eps));
     const double SL = smin(uL - aL, u_roe - c_roe);
     const double SR = smax(uR + aR, u_roe + c_roe);
@@ -133,8 +139,9 @@ __device__ __forceinline__ void hllc_dg_f64(const double L[4], const double R[4]
     const double FF0 = rho * u, FF1 = rho * u * u + p, FF2 = rho * u * v, FF3 = (U3 + p) * u;
     double denom = S - Sstar;
     if (fabs(denom) < eps) denom = (denom >= 0 ? eps : -eps);
-    const double c = rho * (S - u) / denom;
-    double term = (Sstar - u) * (Sstar + p / (rho * (S - u)));
+    const double q = rho * (S - u);
+    const double c = q / denom;
+    double term = (Sstar - u) * (Sstar + p / q);
     if (isnan(term) || isinf(term)) term = 0.0;
     F[0] = FF0 + S * (c - U0);
     F[1] = FF1 + S * (c * Sstar - U1);
