@@ -35,6 +35,8 @@ WORKLOADS = {
     "dg2_tvb_riemann1024": dict(method="dg", P=2, n=1024, limiter="tvb", prec="fp64"),
     "weno_fp64_riemann256": dict(method="weno", n=256, prec="fp64"),
     "weno_fp64_riemann1024": dict(method="weno", n=1024, prec="fp64"),
+    "weno_fp64_jet2048x1024": dict(method="weno", problem="jet", prec="fp64"),
+    "weno_fp32_jet2048x1024": dict(method="weno", problem="jet", prec="fp32"),
 }
 
 
@@ -49,23 +51,39 @@ def build_solver(w: dict):
 
     from zuikakucfd_amd import meshio, zk
 
-    m = meshio.riemann_2d(w["n"])
+    m = meshio.jet() if w.get("problem") == "jet" else meshio.riemann_2d(w["n"])
     if w["method"] == "dg":
         lim = {"tvb": zk.LIMITER_TVB, "persson": zk.LIMITER_PERSSON_ZS, "none": zk.LIMITER_NONE}[w["limiter"]]
         s = zk.Solver(method=zk.METHOD_DG, P=w["P"], limiter=lim, tvb_M=50.0)
         nm = (w["P"] + 1) ** 2
     else:
         prec = zk.PREC_FP64 if w["prec"] == "fp64" else zk.PREC_FP32_BULK
-        s = zk.Solver(method=zk.METHOD_WENO, precision=prec)
+        # the jet needs the face positivity check (the reference scheme loses positivity on it)
+        s = zk.Solver(method=zk.METHOD_WENO, precision=prec, positivity=w.get("problem") == "jet")
         nm = 1
     s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
     lam = s.max_speed()
     if w["method"] == "dg":
         cfl = 0.9 / (2.0 * (2 * w["P"] + 1))
     else:
-        cfl = 0.5
+        cfl = 0.25 if w.get("problem") == "jet" else 0.5
     dt = cfl * m.da / lam
     return s, m, nm, dt
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` (a profiling label such as dg_fused)
+    for `workload`, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/traffic.json, copy-calibrated), or None."""
+    try:
+        d = json.loads((ROOT / "profiles" / "traffic.json").read_text())[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    hits = [v for k, v in d["kernels"].items() if k.startswith(kernel + "_") or k.startswith(kernel + "<")]
+    if len(hits) != 1:
+        return None
+    return {"bytes": hits[0]["bytes"], "read_bytes": hits[0]["read_bytes"], "write_bytes": hits[0]["write_bytes"],
+            "source": "profiles/" + d["source"]}
 
 
 def cpu_baseline(w: dict, m, dt: float, budget_s: float = 20.0) -> dict:
@@ -82,7 +100,8 @@ def cpu_baseline(w: dict, m, dt: float, budget_s: float = 20.0) -> dict:
         lim = {"tvb": 2, "persson": 1, "none": 0}[w["limiter"]]
         o = oracle.DG.from_prims(w["P"], m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype, limiter=lim, tvb_M=50.0)
     else:
-        o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype)
+        o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype,
+                      flux=oracle.POSITIVITY if w.get("problem") == "jet" else 0)
     steps = 0
     t0 = time.perf_counter()
     while True:
@@ -320,18 +339,24 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: four-quadrant Riemann initial data generated in-process",
+        "dtype": "f64" if w["prec"] == "fp64" else "f32 bulk, f64 state",
+        "data": "synthetic: " + ("Mach-80 jet inflow (2048x1024, gamma 5/3)" if w.get("problem") == "jet"
+                                 else "four-quadrant Riemann initial data") + " generated in-process",
         "config": {"workload": args.workload, "method": w["method"], "P": w.get("P"), "grid": [m.ny, m.nx],
                    "updated_cells": nupd, "limiter": w.get("limiter"), "precision": w["prec"], "dt": dt,
                    "parallelism": "single GPU, whole domain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                     "kernel_avg_ms": avg_ms, "algo_bytes_per_cell_stage": bps},
+                     "kernel_avg_ms": avg_ms, "algo_bytes_per_cell_stage": bps,
+                     "algo_bytes_per_launch": bps * nupd},
         "stage_hbm_frac": rate * bps / (HBM_PEAK_GBS * 1e9),
         "kernels": rep,
         "finite": finite,
     }
+    tr = measured_traffic(args.workload, dom)
+    if tr:
+        line["roofline"]["traffic"] = tr["bytes"]
+        line["roofline"]["traffic_detail"] = tr
     if not args.no_paths:
         line["paths"] = {"weno_fp64_riemann256": path_weno_riemann256(4 * args.path_steps),
                          "jet_fp32_bulk": path_jet(args.path_steps),
