@@ -217,14 +217,20 @@ def path_ensemble(steps: int, ninst: int = 256) -> dict:
     for s in solvers:
         s.step_cfl(0.5, 1e30, steps)
     t_loop = time.perf_counter() - t0
+    # Some draws form a vacuum (a strong rarefaction between diverging
+    # quadrants) within the run; the reference scheme, face positivity check
+    # included, is not positivity-preserving there and goes non-finite in
+    # FP64 as well.  Errors are taken over instances finite in both runs.
     errs = []
-    nonfinite = 0
+    nonfinite, nonfinite64 = 0, 0
     for k in range(ninst):
         U, _ = E2.get(k)
-        if not np.isfinite(U).all():
-            nonfinite += 1
-            continue
-        errs.append(_rel_l1(U[0], solvers[k].get_conserved()[0]))
+        U64 = solvers[k].get_conserved()
+        bad32, bad64 = not np.isfinite(U).all(), not np.isfinite(U64).all()
+        nonfinite += bad32
+        nonfinite64 += bad64
+        if not (bad32 or bad64):
+            errs.append(_rel_l1(U[0], U64[0]))
     for s in solvers:
         s.close()
     E2.close()
@@ -233,6 +239,7 @@ def path_ensemble(steps: int, ninst: int = 256) -> dict:
     return {"config": f"{ninst} WENO5 four-quadrant problems, N in {{128..512}}, {ncell} updated cells, CFL 0.5",
             "steps": steps, "fp32_batched_cell_stage_per_s": r_ens, "fp64_loop_cell_stage_per_s": r_loop,
             "speedup": r_ens / r_loop, "l1_rho_vs_fp64_max": max(errs), "nonfinite_instances": nonfinite,
+            "nonfinite_instances_fp64_loop": nonfinite64,
             "positivity_check": True, "tile": tile, "autotune_ms": tune_ms,
             "timing": "wall clock around the launches with a device sync at each end"}
 

@@ -166,9 +166,13 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
 
 /* Kernel-variant selection for the autotuner (no reference counterpart):
  * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
- * default, 2/3/4 = minimum waves per SIMD of the P=3 MFMA cell kernel,
- * -1 = VALU cell kernel. */
-zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant);
+ * automatic (fused row-marching stage kernel at P = 3 without the Persson
+ * limiter, split face/cell kernels otherwise), 10 + R = fused kernel with
+ * R-row segments (R in 4..64, any P), 2/3/4 = split kernels with the P=3
+ * MFMA cell kernel at that minimum of waves per SIMD, -1 = split kernels
+ * with the VALU cell kernel; use_graph = 1 (default) lets zk_step replay
+ * a captured WENO step as a HIP graph. */
+zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph);
 
 /* ---------------------------------------------------------------------------
  * Learned troubled-cell indicator for DG P=2 (no reference counterpart).

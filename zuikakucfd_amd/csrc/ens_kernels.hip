@@ -70,6 +70,7 @@ __device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I)
     A.ca = E.ca; A.cb = E.cb;
     A.stage = E.stage; A.flux = E.flux; A.positivity = E.positivity;
     A.tiles_x = I.tiles_x;
+    A.ntiles = 0; A.ring = nullptr; A.nring = 0;  // the ensemble has its own ring launch
     return A;
 }
 

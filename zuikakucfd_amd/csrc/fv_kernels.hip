@@ -3,8 +3,8 @@
 // One launch = one SSP-RK3 stage over the whole grid:
 //   * fv_stage_kernel: one workgroup per 2-D tile of the update range
 //     [3,ny-3)x[3,nx-3) (fv_tile.h), tiles mapped XCD-contiguously.
-//   * fv_ring_kernel: the cells outside the update range (ghost ring, fixed
-//     inflow cells) get the same combination with dU = 0.
+//   * the blocks after the tiles give the cells outside the update range
+//     (ghost ring, fixed inflow cells) the same combination with dU = 0.
 // The per-face arithmetic follows weno5_reconstruct and hllcFlux of
 // src_WENO/fluid.cpp:230-317; the stage combination follows :468-493.
 #include "fv_tile.h"
@@ -14,36 +14,37 @@ namespace zk {
 
 namespace {
 
+// Cells outside [3,ny-3)x[3,nx-3) (ghost ring, fixed inflow cells): the stage
+// combination with dU = 0, i.e. the stale-ghost mixture of the reference.
+__device__ __forceinline__ void fv_ring_cells(const FvArgs &A, int b)
+{
+    const int t = b * FV_THREADS + threadIdx.x;
+    if (t >= A.nring) return;
+    const size_t k = (size_t)A.ring[t];
+    const int32_t e = A.src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double un = A.Un[v * A.np + k];
+        double uf = A.Uin[v * A.np + s];
+        if ((v == 1 && (e & 1)) || (v == 2 && (e & 2))) uf = -uf;
+        A.Uout[v * A.np + k] = (A.stage == 0) ? un : A.ca * un + A.cb * (uf + 0.0);
+    }
+}
+
+// Blocks [0, ntiles) update one tile each (XCD-aware order); the blocks after
+// them combine the ghost ring, so a stage is a single launch.
 template <int TX, int TY, bool F32>
 __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
 {
     __shared__ double s_u[4][TY + 6][TX + 6];
     __shared__ double s_f[4][(TX + 1) * (TY + 1)];
     if (A.S->done) return;
-    fv_tile_body<TX, TY, F32>(A, xcd_tile(blockIdx.x, gridDim.x), s_u, s_f);
-}
-
-// Cells outside [3,ny-3)x[3,nx-3): combination with dU = 0.
-__global__ void fv_ring_kernel(const double *__restrict__ Un, const double *__restrict__ Uin,
-                               double *__restrict__ Uout, const int32_t *__restrict__ src,
-                               const int32_t *__restrict__ ring, int nring, size_t np, int nplanes,
-                               double ca, double cb, int stage, int neg_c1, int neg_c2,
-                               const StepScalars *__restrict__ S)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nring || S->done) return;
-    const size_t k = (size_t)ring[t];
-    const int32_t e = src[k];
-    const size_t s = (size_t)(e >> kSrcShift);
-    const int per_var = nplanes / 4;
-    for (int p = 0; p < nplanes; ++p) {
-        const int var = p / per_var;
-        const double un = Un[p * np + k];
-        double uf = Uin[p * np + s];
-        if (var == 1 && (e & neg_c1)) uf = -uf;
-        if (var == 2 && (e & neg_c2)) uf = -uf;
-        Uout[p * np + k] = (stage == 0) ? un : ca * un + cb * (uf + 0.0);
+    if ((int)blockIdx.x >= A.ntiles) {
+        fv_ring_cells(A, (int)blockIdx.x - A.ntiles);
+        return;
     }
+    fv_tile_body<TX, TY, F32>(A, xcd_tile(blockIdx.x, A.ntiles), s_u, s_f);
 }
 
 // max(|u| + c) over every cell (ghosts included), no floors:
@@ -80,28 +81,28 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     A.flux = H.cfg.flux;
     A.positivity = H.cfg.positivity;
     const int ux = H.g.nx - 6, uy = H.g.ny - 6;
-    if (ux > 0 && uy > 0) {
-        ProfScope ps(H, "fv_stage");
-        const bool f32 = H.cfg.precision == ZK_PREC_FP32_BULK;
-        // tile shape: 0 = automatic (16x15 for small grids, 32x15 otherwise), 1 = 16x15, 2 = 32x15
-        int shape = H.fv_tile;
-        if (shape == 0) shape = ((size_t)ux * uy < 400000) ? 1 : 2;
-        if (shape == 1) {
-            A.tiles_x = (ux + 15) / 16;
-            const int nb = A.tiles_x * ((uy + 14) / 15);
-            if (f32) hipLaunchKernelGGL((fv_stage_kernel<16, 15, true>), dim3(nb), dim3(FV_THREADS), 0, H.stream, A);
-            else hipLaunchKernelGGL((fv_stage_kernel<16, 15, false>), dim3(nb), dim3(FV_THREADS), 0, H.stream, A);
-        } else {
-            A.tiles_x = (ux + 31) / 32;
-            const int nb = A.tiles_x * ((uy + 14) / 15);
-            if (f32) hipLaunchKernelGGL((fv_stage_kernel<32, 15, true>), dim3(nb), dim3(FV_THREADS), 0, H.stream, A);
-            else hipLaunchKernelGGL((fv_stage_kernel<32, 15, false>), dim3(nb), dim3(FV_THREADS), 0, H.stream, A);
-        }
-    }
-    if (H.nring > 0) {
-        ProfScope ps(H, "fv_ring");
-        hipLaunchKernelGGL(fv_ring_kernel, dim3((H.nring + 255) / 256), dim3(256), 0, H.stream, Un, Uin, Uout,
-                           H.src, H.ring, H.nring, H.g.np, 4, A.ca, A.cb, stage, 1, 2, H.scal);
+    A.ring = H.ring;
+    A.nring = H.nring;
+    const int nrb = (H.nring + FV_THREADS - 1) / FV_THREADS;
+    const bool f32 = H.cfg.precision == ZK_PREC_FP32_BULK;
+    // tile shape: 0 = automatic (16x15 for small grids, 32x15 otherwise), 1 = 16x15, 2 = 32x15
+    int shape = H.fv_tile;
+    if (shape == 0) shape = ((size_t)ux * uy < 400000) ? 1 : 2;
+    const bool tiles = ux > 0 && uy > 0;
+    if (!tiles && nrb == 0) return;
+    ProfScope ps(H, "fv_stage");
+    if (shape == 1) {
+        A.tiles_x = tiles ? (ux + 15) / 16 : 0;
+        A.ntiles = tiles ? A.tiles_x * ((uy + 14) / 15) : 0;
+        const dim3 grid(A.ntiles + nrb);
+        if (f32) hipLaunchKernelGGL((fv_stage_kernel<16, 15, true>), grid, dim3(FV_THREADS), 0, H.stream, A);
+        else hipLaunchKernelGGL((fv_stage_kernel<16, 15, false>), grid, dim3(FV_THREADS), 0, H.stream, A);
+    } else {
+        A.tiles_x = tiles ? (ux + 31) / 32 : 0;
+        A.ntiles = tiles ? A.tiles_x * ((uy + 14) / 15) : 0;
+        const dim3 grid(A.ntiles + nrb);
+        if (f32) hipLaunchKernelGGL((fv_stage_kernel<32, 15, true>), grid, dim3(FV_THREADS), 0, H.stream, A);
+        else hipLaunchKernelGGL((fv_stage_kernel<32, 15, false>), grid, dim3(FV_THREADS), 0, H.stream, A);
     }
 }
 

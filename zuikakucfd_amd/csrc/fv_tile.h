@@ -32,6 +32,9 @@ struct FvArgs {
     int flux;
     int positivity;
     int tiles_x;
+    int ntiles;          // blocks past ntiles combine the ghost ring instead
+    const int32_t *ring; // cells outside the update range
+    int nring;
 };
 
 // Face positivity check in FP64 (an option with no reference counterpart).

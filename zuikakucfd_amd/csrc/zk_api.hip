@@ -78,8 +78,19 @@ zk_status check_args(Handle *H)
     return ZK_OK;
 }
 
+// Drops the captured step (its kernel arguments name the current buffers and
+// tuning, so it is rebuilt after any change to them).
+void drop_graph(Handle *H)
+{
+    if (H->step_exec) (void)hipGraphExecDestroy(H->step_exec);
+    if (H->step_graph) (void)hipGraphDestroy(H->step_graph);
+    H->step_exec = nullptr;
+    H->step_graph = nullptr;
+}
+
 void free_all(Handle *H)
 {
+    drop_graph(H);
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
                     H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand};
     for (void *p : ptrs)
@@ -423,6 +434,26 @@ zk_status zk_step(zk_handle *h, double dt, int nsteps)
     }
     ZK_HIP(hipSetDevice(H->cfg.device));
     hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, dt);
+    // WENO steps are short enough for launch overhead to matter on small grids:
+    // one step (dt is read from device memory) is captured once and replayed.
+    // DG stages are long, and the learned limiter salts its hashes per stage.
+    if (H->cfg.method == ZK_METHOD_WENO && H->use_graph && !H->prof && nsteps >= 2) {
+        if (!H->step_exec) {
+            ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
+            st = enqueue_step(H);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(H->stream, &g);
+            if (st) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            ZK_HIP(e);
+            H->step_graph = g;
+            ZK_HIP(hipGraphInstantiate(&H->step_exec, g, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; ++s) ZK_HIP(hipGraphLaunch(H->step_exec, H->stream));
+        return ZK_OK;
+    }
     for (int s = 0; s < nsteps; ++s) {
         st = enqueue_step(H);
         if (st) return st;
@@ -491,7 +522,11 @@ zk_status zk_step_timed(zk_handle *h, double dt, int nsteps, float *ms)
     ZK_HIP(hipEventCreate(&e1));
     ZK_HIP(hipEventRecord(e0, H->stream));
     st = zk_step(h, dt, nsteps);
-    if (st) return st;
+    if (st) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return st;
+    }
     ZK_HIP(hipEventRecord(e1, H->stream));
     ZK_HIP(hipEventSynchronize(e1));
     float t = 0.f;
@@ -569,17 +604,21 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len)
     return ZK_OK;
 }
 
-zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant)
+zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph)
 {
     Handle *H = reinterpret_cast<Handle *>(h);
     if (!H) return ZK_ERR_ARG;
-    if (fv_tile < 0 || fv_tile > 2 || dg_variant < -1 || dg_variant > 4) {
-        H->err = "zk_set_tuning: fv_tile in 0..2, dg_variant in -1..4";
+    const bool dg_ok = dg_variant == -1 || dg_variant == 0 || (dg_variant >= 2 && dg_variant <= 4) ||
+                       (dg_variant >= 14 && dg_variant <= 74);
+    if (fv_tile < 0 || fv_tile > 2 || !dg_ok) {
+        H->err = "zk_set_tuning: fv_tile in 0..2, dg_variant -1, 0, 2..4 or 10+R with R in 4..64";
         return ZK_ERR_ARG;
     }
+    drop_graph(H);
     H->fv_tile = fv_tile;
     H->no_mfma = dg_variant == -1;
     H->tune = dg_variant > 0 ? dg_variant : 0;
+    H->use_graph = use_graph != 0;
     return ZK_OK;
 }
 

@@ -46,6 +46,9 @@ struct Handle {
     bool no_mfma = false;     // force the VALU cell kernel (ZK_NO_MFMA=1), for comparison
     int tune = 0;             // kernel variant selector (ZK_TUNE), used by the autotuner
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
+    bool use_graph = true;    // WENO zk_step: replay one captured step as a HIP graph
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
     int64_t n_updated = 0;
 
     // device buffers

@@ -88,7 +88,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_updated_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.zk_profile.argtypes = [vp, C.c_int]
     lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
-    lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.zk_ensemble_create.argtypes = [C.POINTER(ZkConfig), C.POINTER(vp)]
     lib.zk_ensemble_add.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, ip, C.c_double]
     lib.zk_ensemble_finalize.argtypes = [vp]
@@ -234,8 +234,9 @@ class Solver:
         self._check(self.lib.zk_step_timed(self.h, float(dt), int(nsteps), C.byref(ms)), "zk_step_timed")
         return ms.value
 
-    def set_tuning(self, fv_tile: int = 0, dg_variant: int = 0) -> None:
-        self._check(self.lib.zk_set_tuning(self.h, int(fv_tile), int(dg_variant)), "zk_set_tuning")
+    def set_tuning(self, fv_tile: int = 0, dg_variant: int = 0, use_graph: bool = True) -> None:
+        self._check(self.lib.zk_set_tuning(self.h, int(fv_tile), int(dg_variant), 1 if use_graph else 0),
+                    "zk_set_tuning")
 
     def attach_mlp(self, mlp: "Mlp", lo: float = 0.2, hi: float = 0.8, audit: float = 0.02) -> None:
         self._mlp = mlp  # keep the network alive while the handle uses it
