@@ -42,6 +42,25 @@ def test_zk_weno_fixed_dt(tmp_path):
         np.testing.assert_allclose(txt.reshape(64, 64), o.U[c], rtol=1e-5, atol=1e-12)
 
 
+def test_zk_weno_rank_files(tmp_path):
+    """--ranks 4 writes the per-rank files of `mpirun -np 4`; recombined as
+    plot.ipynb does they equal the single-rank output."""
+    m = meshio.riemann_2d(48)
+    meshio.write_mesh_folder(m, tmp_path / "mesh")
+    for n in (1, 4):
+        d = tmp_path / f"r{n}"
+        d.mkdir()
+        r = run([BIN / "zk_weno", "../mesh", "2e-4", "20", "--quiet", "--ranks", str(n)], d)
+        assert r.returncode == 0, r.stderr
+    one = meshio.combine_rank_files(tmp_path / "r1" / "result" / "20", 1)
+    four = meshio.combine_rank_files(tmp_path / "r4" / "result" / "20", 4)
+    assert (tmp_path / "r4" / "result" / "20" / "U3_3.dat").exists()
+    np.testing.assert_array_equal(four, one)
+    lo, hi = meshio.rank_columns(48, 4)[2]
+    U1 = np.loadtxt(tmp_path / "r4" / "result" / "20" / "U1_2.dat")
+    assert U1.shape == (48, hi - lo)
+
+
 def test_zk_dg_cfl_abort(tmp_path):
     meshio.write_mesh_folder(meshio.riemann_2d(32), tmp_path / "mesh")
     r = run([BIN / "zk_dg4", "mesh", "0.01", "5"], tmp_path)

@@ -61,3 +61,51 @@ def test_smooth_wave_modes_projection(P):
     yfac = 2 * np.sin(k) / k  # int_{-1}^{1} cos-free part of the eta integral
     expect = 0.75 * 0.2 * mom * yfac * np.cos(2 * np.pi * (X + Y))
     np.testing.assert_allclose(Q[0, P + 1], expect, atol=1e-13)
+
+
+def split_reference(M, n):
+    """Direct restatement of splitMeshVertically's column bookkeeping
+    (src_WENO/fluid.cpp:98-152) on one matrix: per rank the real columns,
+    then ghost columns copied from the clamped neighbours."""
+    Ny, Nx = M.shape
+    widths, remain = [], Nx
+    for k in range(n):
+        widths.append(remain // (n - k))
+        remain -= widths[-1]
+    subs, start = [], 0
+    for k in range(n):
+        w = widths[k]
+        lg, rg = (3 if k > 0 else 0), (3 if k < n - 1 else 0)
+        sub = np.zeros((Ny, w + lg + rg))
+        for j in range(w):
+            sub[:, lg + j] = M[:, start + j]
+        for gj in range(lg):
+            sub[:, gj] = M[:, max(start - lg + gj, 0)]
+        for gj in range(rg):
+            sub[:, lg + w + gj] = M[:, min(start + w + gj, Nx - 1)]
+        subs.append(sub)
+        start += w
+    return subs
+
+
+@pytest.mark.parametrize("nx,n", [(256, 4), (100, 3), (67, 5), (30, 1), (41, 2)])
+def test_rank_columns_match_split(nx, n):
+    """Each rank's slice of a marker matrix equals the reference's sub-mesh
+    (exchanged ghost columns hold the neighbour's values, as after the
+    post-stage column exchange)."""
+    M = np.arange(5 * nx, dtype=float).reshape(5, nx)
+    cols = meshio.rank_columns(nx, n)
+    subs = split_reference(M, n)
+    assert len(cols) == n
+    for (lo, hi), sub in zip(cols, subs):
+        np.testing.assert_array_equal(M[:, lo:hi], sub)
+
+
+def test_rank_files_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    U = rng.normal(size=(4, 7, 40))
+    meshio.save_rank_files(U, tmp_path / "r4", 4)
+    meshio.save_rank_files(U, tmp_path / "r1", 1)
+    assert sorted(p.name for p in (tmp_path / "r4").iterdir())[:4] == ["U0_0.dat", "U0_1.dat", "U0_2.dat", "U0_3.dat"]
+    np.testing.assert_array_equal(meshio.combine_rank_files(tmp_path / "r4", 4),
+                                  meshio.combine_rank_files(tmp_path / "r1", 1))

@@ -19,6 +19,8 @@
 //                            <executable dir>/../data/troubled_cell_p2.f32
 //   --band <lo> <hi>         (learned) score band deferred to the TVB test, default 0.2 0.8
 //   --audit <fraction>       (learned) random share of cells also checked by TVB, default 0.02
+//   --ranks <n>              write U<c>_<r>.dat for r = 0..n-1 as `mpirun -np n` would
+//                            (column slices with 3 exchanged ghost columns per inner side)
 //   --dump                   also write result/<step>/state.f64 (all modes, full precision)
 //   --quiet                  no per-step progress lines
 #include <chrono>
@@ -31,6 +33,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "zk.h"
@@ -87,13 +90,33 @@ bool read_input(const std::string &dir, Input &in, std::string &err)
 // Eigen 3 default operator<< text: stream precision 6 (%g), every coefficient
 // right-aligned to the widest one, single spaces, newline between rows, no
 // trailing newline.
-void write_eigen(const fs::path &path, const double *a, int ny, int nx)
+// Column slices [lo, hi) of the per-rank output of a `mpirun -np n` run:
+// rank widths w_k = remain / (n - k) (splitMeshVertically,
+// src_WENO/fluid.cpp:98-152), each rank's file holding its own columns plus
+// 3 exchanged ghost columns on every inner side.  Interior values do not
+// depend on the rank count, so slicing the whole-domain result reproduces them.
+std::vector<std::pair<int, int>> rank_columns(int nx, int n)
 {
+    std::vector<std::pair<int, int>> out;
+    int remain = nx, start = 0;
+    for (int k = 0; k < n; ++k) {
+        const int w = remain / (n - k);
+        remain -= w;
+        const int lg = k > 0 ? 3 : 0, rg = k < n - 1 ? 3 : 0;
+        out.emplace_back(std::max(start - lg, 0), std::min(start + w + rg, nx));
+        start += w;
+    }
+    return out;
+}
+
+void write_eigen(const fs::path &path, const double *a, int ny, int nx, int ld = -1)
+{
+    if (ld < 0) ld = nx;
     std::vector<std::string> s((size_t)ny * nx);
     size_t w = 0;
     char buf[64];
     for (size_t k = 0; k < s.size(); ++k) {
-        std::snprintf(buf, sizeof(buf), "%g", a[k]);
+        std::snprintf(buf, sizeof(buf), "%g", a[(k / nx) * (size_t)ld + k % nx]);
         s[k] = buf;
         w = std::max(w, s[k].size());
     }
@@ -123,6 +146,7 @@ struct Opts {
     bool quiet = false;
     bool positivity = false;
     std::string weights;
+    int ranks = 1;  // per-rank output files as a `mpirun -np ranks` run would write
     float band_lo = 0.2f, band_hi = 0.8f, audit = 0.02f;
 };
 
@@ -148,6 +172,7 @@ bool parse_opts(int argc, char **argv, Opts &o)
             o.band_lo = std::stof(argv[++a]);
             o.band_hi = std::stof(argv[++a]);
         } else if (k == "--audit" && val(v)) o.audit = std::stof(v);
+        else if (k == "--ranks" && val(v)) o.ranks = std::max(1, std::stoi(v));
         else if (k == "--dump") o.dump = true;
         else if (k == "--quiet") o.quiet = true;
         else if (k == "--positivity") o.positivity = true;
@@ -193,15 +218,18 @@ int fail(zk_handle *h, const char *what)
     return 2;
 }
 
-void save(zk_handle *h, const Input &in, int nm, int step, bool dump)
+void save(zk_handle *h, const Input &in, int nm, int step, bool dump, int ranks)
 {
     const std::string folder = "result/" + std::to_string(step);
     fs::create_directories(folder);
     const size_t n = (size_t)in.nx * in.ny;
     std::vector<double> U(4 * n);
     zk_get_conserved(h, U.data(), U.data() + n, U.data() + 2 * n, U.data() + 3 * n);
-    for (int c = 0; c < 4; ++c)
-        write_eigen(fs::path(folder) / ("U" + std::to_string(c) + "_0.dat"), U.data() + c * n, in.ny, in.nx);
+    const auto cols = rank_columns(in.nx, ranks);
+    for (int r = 0; r < ranks; ++r)
+        for (int c = 0; c < 4; ++c)
+            write_eigen(fs::path(folder) / ("U" + std::to_string(c) + "_" + std::to_string(r) + ".dat"),
+                        U.data() + c * n + cols[r].first, in.ny, cols[r].second - cols[r].first, in.nx);
     if (dump) {
         std::vector<double> Q(4 * (size_t)nm * n);
         zk_get_state(h, Q.data());
@@ -320,7 +348,7 @@ int main(int argc, char **argv)
         }
         if (step % output_interval == 0 || step == timesteps || (time_mode && t >= t_end)) {
             const auto i0 = Clock::now();
-            save(h, in, nm, step, o.dump);
+            save(h, in, nm, step, o.dump, o.ranks);
             io_ms += ms_since(i0);
         }
     }
@@ -386,7 +414,7 @@ int main(int argc, char **argv)
         }
         if (step % output_interval == 0 || step == timesteps || (time_mode && t >= t_end)) {
             const auto i0 = Clock::now();
-            save(h, in, nm, step, o.dump);
+            save(h, in, nm, step, o.dump, o.ranks);
             io_ms += ms_since(i0);
         }
     }

@@ -83,6 +83,40 @@ def save_mesh_data(U: np.ndarray, rank: int, folder: str | os.PathLike) -> None:
         (folder / f"U{c}_{rank}.dat").write_text(eigen_text(U[c]))
 
 
+def rank_columns(nx: int, n: int) -> list[tuple[int, int]]:
+    """Column slices [lo, hi) of the per-rank files of a ``mpirun -np n`` run:
+    widths w_k = remain // (n - k) (splitMeshVertically,
+    src_WENO/fluid.cpp:98-152), plus 3 exchanged ghost columns on every
+    inner side of a rank."""
+    out, remain, start = [], nx, 0
+    for k in range(n):
+        w = remain // (n - k)
+        remain -= w
+        lg, rg = (3 if k > 0 else 0), (3 if k < n - 1 else 0)
+        out.append((max(start - lg, 0), min(start + w + rg, nx)))
+        start += w
+    return out
+
+
+def save_rank_files(U: np.ndarray, folder: str | os.PathLike, n: int) -> None:
+    """Write a whole-domain U0..U3 as the n per-rank files of ``mpirun -np n``."""
+    for r, (lo, hi) in enumerate(rank_columns(U.shape[-1], n)):
+        save_mesh_data(U[..., lo:hi], r, folder)
+
+
+def combine_rank_files(folder: str | os.PathLike, n: int, ghost: int = 3) -> np.ndarray:
+    """Reassemble U0..U3 from n per-rank files, dropping the inner ghost
+    columns (load_and_combine_data of plot.ipynb, with n ranks)."""
+    folder = Path(folder)
+    parts = []
+    for r in range(n):
+        U = np.stack([np.loadtxt(folder / f"U{c}_{r}.dat", ndmin=2) for c in range(4)])
+        lo = ghost if r > 0 else 0
+        hi = U.shape[-1] - (ghost if r < n - 1 else 0)
+        parts.append(U[..., lo:hi])
+    return np.concatenate(parts, axis=-1)
+
+
 def save_binary_state(Q: np.ndarray, t: float, step: int, path: str | os.PathLike) -> None:
     """Full-precision dump of every mode (no reference counterpart)."""
     np.savez(path, Q=Q, t=t, step=step)
