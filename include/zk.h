@@ -60,9 +60,13 @@ typedef struct zk_config {
     int limiter;     /* ZK_LIMITER_*; DG only */
     double tvb_M;    /* TVB constant M (north_star: 50) */
     int device;      /* HIP device ordinal */
-    int positivity;  /* WENO: 1 = face positivity check (FP64): a face whose reconstructed
-                        states have rho or p below 1e-12 falls back to first order.
-                        No reference counterpart; 0 reproduces the reference scheme. */
+    int positivity;  /* WENO: 1 = positivity option (FP64), no reference counterpart;
+                        0 reproduces the reference scheme.  Face check: a face whose
+                        reconstructed states have rho or p below 1e-12 uses the
+                        first-order states.  Cell fallback: an updated cell whose new
+                        state has rho or p not above 1e-12 (or is not finite) is bad, and
+                        every face touching a bad cell takes the first-order flux for the
+                        stage (once; conservative, faces shared). */
 } zk_config;
 
 /* Create a handle.  Replaces the `Mesh` object's lifetime

@@ -34,6 +34,7 @@ def lib() -> C.CDLL:
         L.zko_fv_fill_ghosts.argtypes = [i, i, ip, dp]
         L.zko_fv_rhs.argtypes = [i, i, d, d, ip, dp, d, i, dp]
         L.zko_fv_step.argtypes = [i, i, d, d, ip, dp, d, i]
+        L.zko_fv_step.restype = i
         L.zko_fv_max_speed.argtypes = [i, i, d, dp]
         L.zko_fv_max_speed.restype = d
         L.zko_prim_to_cons.argtypes = [i, d, dp, dp, dp, dp, dp]
@@ -90,17 +91,20 @@ POSITIVITY = 16
 
 class FV:
     """Finite-volume oracle state: U has shape (4, ny, nx).  flux may be or-ed
-    with POSITIVITY to apply the face positivity fallback."""
+    with POSITIVITY to apply the face positivity check and the cell
+    positivity fallback; `fallbacks` counts the bad cells the fallback met."""
 
     def __init__(self, da, gamma, U, bctype, flux: int = 0):
         self.U = np.ascontiguousarray(U, dtype=np.float64).copy()
         self.bct = np.ascontiguousarray(bctype, dtype=np.int32)
         self.ny, self.nx = self.bct.shape
         self.da, self.gamma, self.flux = float(da), float(gamma), flux
+        self.fallbacks = 0
 
     def step(self, dt: float, n: int = 1) -> None:
         for _ in range(n):
-            lib().zko_fv_step(self.ny, self.nx, self.da, self.gamma, self.bct, self.U, float(dt), self.flux)
+            self.fallbacks += lib().zko_fv_step(self.ny, self.nx, self.da, self.gamma, self.bct, self.U, float(dt),
+                                                self.flux)
 
     def rhs(self, dt: float) -> np.ndarray:
         dU = np.zeros_like(self.U)

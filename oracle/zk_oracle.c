@@ -348,8 +348,11 @@ static inline void positivity_fix(double QL[4], double QR[4], const double cl[4]
     }
 }
 
-void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
-                double *U, double dt, int flux, double *dU)
+/* As zko_fv_rhs; when bad is non-null, every face with a bad cell on either
+ * side uses the first-order states (the two cell values).  This is the second
+ * pass of the cell positivity fallback of zko_fv_step. */
+void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *bct,
+                       double *U, double dt, int flux, double *dU, const uint8_t *bad)
 {
     const size_t NP = (size_t)ny * nx;
     const double dx = da, dy = da;
@@ -389,6 +392,17 @@ void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
                 positivity_fix(QLd, QRd, Y[3], Y[4], gamma);
                 positivity_fix(QLu, QRu, Y[2], Y[3], gamma);
             }
+            if (bad) {
+                const int b0 = bad[IDX(i, j)];
+                if (b0 || bad[IDX(i, j + 1)])
+                    for (int c = 0; c < 4; ++c) { QLr[c] = X[3][c]; QRr[c] = X[4][c]; }
+                if (b0 || bad[IDX(i, j - 1)])
+                    for (int c = 0; c < 4; ++c) { QLl[c] = X[2][c]; QRl[c] = X[3][c]; }
+                if (b0 || bad[IDX(i + 1, j)])
+                    for (int c = 0; c < 4; ++c) { QLd[c] = Y[3][c]; QRd[c] = Y[4][c]; }
+                if (b0 || bad[IDX(i - 1, j)])
+                    for (int c = 0; c < 4; ++c) { QLu[c] = Y[2][c]; QRu[c] = Y[3][c]; }
+            }
             double Fr[4], Fl[4], Fd[4], Fu[4];
             fl(QLr, QRr, gamma, Fr);
             fl(QLl, QRl, gamma, Fl);
@@ -408,27 +422,75 @@ void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
     }
 }
 
-/* One SSP-RK3 step in Shu-Osher form over every cell (ghosts included, which
- * leaves the stale ghost mixtures of the reference).  src_WENO/fluid.cpp:468-493. */
-void zko_fv_step(int ny, int nx, double da, double gamma, const int32_t *bct,
-                 double *U, double dt, int flux)
+void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
+                double *U, double dt, int flux, double *dU)
 {
-    const size_t N4 = 4 * (size_t)ny * nx;
+    zko_fv_rhs_masked(ny, nx, da, gamma, bct, U, dt, flux, dU, NULL);
+}
+
+/* Shu-Osher combination of one stage: stage 0 T = Un + dU, else
+ * T = ca Un + cb (U + dU), over every cell. */
+static void fv_combine(size_t N4, int stage, const double *Un, const double *U, const double *dU, double *T)
+{
+    if (stage == 0)
+        for (size_t k = 0; k < N4; ++k) T[k] = Un[k] + dU[k];
+    else if (stage == 1)
+        for (size_t k = 0; k < N4; ++k) T[k] = 0.75 * Un[k] + 0.25 * (U[k] + dU[k]);
+    else
+        for (size_t k = 0; k < N4; ++k) T[k] = (1.0 / 3.0) * Un[k] + (2.0 / 3.0) * (U[k] + dU[k]);
+}
+
+/* One SSP-RK3 step in Shu-Osher form over every cell (ghosts included, which
+ * leaves the stale ghost mixtures of the reference).  src_WENO/fluid.cpp:468-493.
+ *
+ * With ZKO_POSITIVITY (an option of this project, no reference counterpart)
+ * each stage also has a cell positivity fallback: an updated cell whose new
+ * state has rho or p not above 1e-12 (or is not finite) is bad, and the stage
+ * is recomputed with first-order fluxes at every face touching a bad cell
+ * (once; the cells away from bad cells keep their values).  Returns the
+ * number of bad cells met in the three stages. */
+int zko_fv_step(int ny, int nx, double da, double gamma, const int32_t *bct,
+                double *U, double dt, int flux)
+{
+    int nbad = 0;
+    const size_t NP = (size_t)ny * nx, N4 = 4 * NP;
+    const int pos = (flux & ZKO_POSITIVITY) != 0;
     double *Un = (double *)malloc(N4 * sizeof(double));
     double *dU = (double *)malloc(N4 * sizeof(double));
+    double *T = (double *)malloc(N4 * sizeof(double));
+    uint8_t *bad = pos ? (uint8_t *)calloc(NP, 1) : NULL;
     memcpy(Un, U, N4 * sizeof(double));
-
-    zko_fv_rhs(ny, nx, da, gamma, bct, U, dt, flux, dU);
-    for (size_t k = 0; k < N4; ++k) U[k] = Un[k] + dU[k];
-
-    zko_fv_rhs(ny, nx, da, gamma, bct, U, dt, flux, dU);
-    for (size_t k = 0; k < N4; ++k) U[k] = 0.75 * Un[k] + 0.25 * (U[k] + dU[k]);
-
-    zko_fv_rhs(ny, nx, da, gamma, bct, U, dt, flux, dU);
-    for (size_t k = 0; k < N4; ++k) U[k] = (1.0 / 3.0) * Un[k] + (2.0 / 3.0) * (U[k] + dU[k]);
-
+    for (int stage = 0; stage < 3; ++stage) {
+        zko_fv_rhs(ny, nx, da, gamma, bct, U, dt, flux, dU);
+        fv_combine(N4, stage, Un, U, dU, T);
+        if (pos) {
+            int any = 0;
+            for (int i = 3; i < ny - 3; ++i)
+                for (int j = 3; j < nx - 3; ++j) {
+                    const size_t k = IDX(i, j);
+                    int b = 0;
+                    if (bct[k] == 0) {
+                        /* p > 1e-12 multiplied through by 2 rho / (gamma - 1) > 0 */
+                        const double rho = T[k];
+                        const double q = 2.0 * rho * T[3 * NP + k] - (T[NP + k] * T[NP + k] + T[2 * NP + k] * T[2 * NP + k]);
+                        b = !(rho > 1e-12 && q > 2.0 * rho * 1e-12 / (gamma - 1.0));
+                    }
+                    bad[k] = (uint8_t)b;
+                    any |= b;
+                    nbad += b;
+                }
+            if (any) {
+                zko_fv_rhs_masked(ny, nx, da, gamma, bct, U, dt, flux, dU, bad);
+                fv_combine(N4, stage, Un, U, dU, T);
+            }
+        }
+        memcpy(U, T, N4 * sizeof(double));
+    }
     free(Un);
     free(dU);
+    free(T);
+    free(bad);
+    return nbad;
 }
 
 /* max(|u| + c) over every cell, ghosts included, no floors.

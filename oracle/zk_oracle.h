@@ -43,9 +43,11 @@ void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamm
  * or-ed with ZKO_POSITIVITY it also applies the face positivity fallback. */
 #define ZKO_POSITIVITY 16
 void zko_fv_fill_ghosts(int ny, int nx, const int32_t *bct, double *U /*4 planes*/);
+void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *bct,
+                       double *U, double dt, int flux, double *dU, const uint8_t *bad);
 void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,
                 double *U, double dt, int flux, double *dU);
-void zko_fv_step(int ny, int nx, double da, double gamma, const int32_t *bct,
+int zko_fv_step(int ny, int nx, double da, double gamma, const int32_t *bct,
                  double *U, double dt, int flux);
 double zko_fv_max_speed(int ny, int nx, double gamma, const double *U);
 void zko_prim_to_cons(int n, double gamma, const double *rho, const double *u,

@@ -333,6 +333,80 @@ def test_ensemble_matches_single_grids():
             assert rel_l1(U[c], Us[c]) <= TOL, (k, c)
 
 
+def diverging_mesh(n=46):
+    """Periodic flow that opens a vacuum (see tests/test_positivity.py)."""
+    m = meshio.riemann_2d(64)
+    x = (np.arange(n) - 3 + 0.5) / (n - 6)
+    X, Y = np.meshgrid(x, x)
+    m.rho = np.ones((n, n)); m.p = np.full((n, n), 0.1)
+    u = np.where(X < 0.5, -3.0, 3.0)
+    m.u = np.where((X < 0.25) | (X > 0.75), -u, u)
+    m.v = 0.3 * np.sin(2 * np.pi * Y)
+    m.bctype = meshio.ring_bctype(n, n, -2)
+    m.nx = m.ny = n
+    m.da = 1.0 / (n - 6)
+    m.gamma = 1.4
+    return m
+
+
+def test_weno_cell_fallback_parity():
+    """Positivity option through a vacuum: face check and cell fallback
+    (first-order faces around bad cells) match the oracle step by step."""
+    say("weno cell positivity fallback, diverging 46^2")
+    m = diverging_mesh()
+    s = zk.Solver(method=zk.METHOD_WENO, positivity=True)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype,
+                  flux=oracle.POSITIVITY)
+    for _ in range(12):
+        dt = 0.4 * m.da / o.max_speed()
+        s.step(dt, 1)
+        o.step(dt)
+    assert o.fallbacks > 0
+    U = s.get_conserved()
+    assert np.isfinite(U).all()
+    for c in range(4):
+        assert rel_l1(U[c], o.U[c]) <= TOL, c
+
+
+def test_weno_fp32_cell_fallback_finite():
+    """FP32 bulk through the same vacuum: finite, density close to FP64."""
+    say("weno fp32 cell positivity fallback")
+    m = diverging_mesh()
+    s = zk.Solver(method=zk.METHOD_WENO, precision=zk.PREC_FP32_BULK, positivity=True)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype,
+                  flux=oracle.POSITIVITY)
+    for _ in range(20):
+        dt = 0.4 * m.da / o.max_speed()
+        s.step(dt, 1)
+        o.step(dt)
+    U = s.get_conserved()
+    assert np.isfinite(U).all()
+    assert rel_l1(U[0], o.U[0]) <= 1e-3
+
+
+def test_ensemble_cell_fallback_matches_single_grids():
+    """An ensemble member that opens a vacuum, next to an ordinary one, equals
+    the single-grid run with the same positivity option."""
+    say("ensemble cell fallback")
+    ms = [diverging_mesh(), _ensemble_members(100, [40])[0]]
+    E = zk.Ensemble(positivity=True)
+    for m in ms:
+        E.add(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype, cfl=0.4)
+    E.finalize()
+    E.step(14)
+    for k, m in enumerate(ms):
+        U, _ = E.get(k)
+        s = zk.Solver(method=zk.METHOD_WENO, positivity=True)
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+        s.step_cfl(0.4, 1e30, 14)
+        Us = s.get_conserved()
+        assert np.isfinite(U).all(), k
+        for c in range(4):
+            assert rel_l1(U[c], Us[c]) <= TOL, (k, c)
+
+
 def test_ensemble_graph_replays_match_eager():
     """Replays of the captured step, with host reads between them, equal the
     eager launch sequence at every checkpoint."""

@@ -55,9 +55,12 @@ struct EnsArgs {
     int stage;
     double ca, cb;
     int flux, positivity;
+    uint8_t *bad;            // cell positivity fallback: per-cell flags (packed), or null
+    unsigned int *any;       // bad-cell counts, one per stage
+    int2 *blist;             // the bad cells of the stage: (instance, local cell)
 };
 
-__device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I)
+__device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I, int idx)
 {
     FvArgs A;
     A.Un = E.Un + I.off;
@@ -71,6 +74,10 @@ __device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I)
     A.stage = E.stage; A.flux = E.flux; A.positivity = E.positivity;
     A.tiles_x = I.tiles_x;
     A.ntiles = 0; A.ring = nullptr; A.nring = 0;  // the ensemble has its own ring launch
+    A.bad = E.bad ? E.bad + I.cell_off : nullptr;
+    A.any = E.any ? E.any + E.stage : nullptr;
+    A.blist = E.blist;
+    A.inst = idx;
     return A;
 }
 
@@ -82,10 +89,26 @@ __global__ __launch_bounds__(zk::FV_THREADS) void ens_stage_kernel(EnsArgs E)
     for (int w = blockIdx.x; w < E.nwork; w += gridDim.x) {
         const int2 it = E.work[w];
         const EnsInst I = E.inst[it.x];
-        FvArgs A = make_args(E, I);
+        FvArgs A = make_args(E, I, it.x);
         A.S = E.S + it.x;
         zk::fv_tile_body<TX, TY, F32>(A, it.y, s_u, s_f);
         __syncthreads();  // the next item restages the LDS tile
+    }
+}
+
+// Cell positivity fallback, pass 2: a small grid-strided launch over the bad
+// cells pass 1 listed, (instance, cell); ends at once when there are none.
+template <bool F32>
+__global__ __launch_bounds__(256) void ens_fallback_kernel(EnsArgs E)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) E.any[(E.stage + 1) % 3] = 0u;
+    const unsigned int n = E.any[E.stage];
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int2 b = E.blist[t];
+        const EnsInst I = E.inst[b.x];
+        FvArgs A = make_args(E, I, b.x);
+        A.S = E.S + b.x;
+        zk::fv_fallback_bad<F32>(A, (size_t)b.y);
     }
 }
 
@@ -174,6 +197,9 @@ struct zk_ensemble {
     int2 *d_work = nullptr, *d_ring = nullptr, *d_chunks = nullptr;
     StepScalars *S = nullptr;
     unsigned long long *lam = nullptr;
+    uint8_t *bad = nullptr;       // cell positivity fallback (positivity on)
+    unsigned int *any = nullptr;
+    int2 *blist = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int persistent_blocks = 0;
@@ -205,6 +231,9 @@ EnsArgs args_for(zk_ensemble *E, const double *Un, const double *Uin, double *Uo
     A.cb = stage == 1 ? 0.25 : (2.0 / 3.0);
     A.flux = E->cfg.flux;
     A.positivity = E->cfg.positivity;
+    A.bad = E->bad;
+    A.any = E->any;
+    A.blist = E->blist;
     return A;
 }
 
@@ -228,6 +257,11 @@ void enqueue_ens_step(zk_ensemble *E)
             if (f32) hipLaunchKernelGGL((ens_stage_kernel<16, 15, true>), grid, blk, 0, E->stream, A);
             else hipLaunchKernelGGL((ens_stage_kernel<16, 15, false>), grid, blk, 0, E->stream, A);
         }
+        if (A.bad) {
+            const dim3 fg(64);  // grid-strided over the listed bad cells
+            if (f32) hipLaunchKernelGGL(ens_fallback_kernel<true>, fg, dim3(256), 0, E->stream, A);
+            else hipLaunchKernelGGL(ens_fallback_kernel<false>, fg, dim3(256), 0, E->stream, A);
+        }
         if (A.nring > 0) hipLaunchKernelGGL(ens_ring_kernel, dim3((A.nring + 255) / 256), dim3(256), 0, E->stream, A);
     }
     hipLaunchKernelGGL(ens_advance_kernel, dim3((ninst + 255) / 256), dim3(256), 0, E->stream, A0);
@@ -239,7 +273,8 @@ void free_ens(zk_ensemble *E)
     if (E->graph) (void)hipGraphDestroy(E->graph);
     E->exec = nullptr;
     E->graph = nullptr;
-    void *ptrs[] = {E->q, E->qa, E->qb, E->src, E->upd, E->d_inst, E->d_work, E->d_ring, E->d_chunks, E->S, E->lam};
+    void *ptrs[] = {E->q, E->qa, E->qb, E->src, E->upd, E->d_inst, E->d_work, E->d_ring, E->d_chunks, E->S, E->lam,
+                    E->bad, E->any, E->blist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -365,6 +400,13 @@ zk_status zk_ensemble_finalize(zk_ensemble *E)
     ENS_HIP(hipMalloc(&E->d_chunks, E->chunks.size() * sizeof(int2)));
     ENS_HIP(hipMalloc(&E->S, E->inst.size() * sizeof(StepScalars)));
     ENS_HIP(hipMalloc(&E->lam, E->inst.size() * sizeof(unsigned long long)));
+    if (E->cfg.positivity) {
+        ENS_HIP(hipMalloc(&E->bad, nc));
+        ENS_HIP(hipMemset(E->bad, 0, nc));
+        ENS_HIP(hipMalloc(&E->any, 3 * sizeof(unsigned int)));
+        ENS_HIP(hipMemset(E->any, 0, 3 * sizeof(unsigned int)));
+        ENS_HIP(hipMalloc(&E->blist, nc * sizeof(int2)));
+    }
     ENS_HIP(hipMemcpy(E->q, E->h_state.data(), ns * sizeof(double), hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(E->qa, E->h_state.data(), ns * sizeof(double), hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(E->qb, E->h_state.data(), ns * sizeof(double), hipMemcpyHostToDevice));

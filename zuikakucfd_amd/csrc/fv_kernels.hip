@@ -47,6 +47,20 @@ __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
     fv_tile_body<TX, TY, F32>(A, xcd_tile(blockIdx.x, A.ntiles), s_u, s_f);
 }
 
+// Cell positivity fallback, pass 2 (positivity option on): a small
+// grid-strided launch over the bad cells pass 1 listed, each with the good
+// neighbours it owns.  The counters rotate over the three stages: this launch
+// clears the next stage's.
+template <bool F32>
+__global__ __launch_bounds__(256) void fv_fallback_kernel(FvArgs A, unsigned int *any, int slot)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) any[(slot + 1) % 3] = 0u;
+    if (A.S->done) return;
+    const unsigned int n = any[slot];
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        fv_fallback_bad<F32>(A, (size_t)A.blist[t].y);
+}
+
 // max(|u| + c) over every cell (ghosts included), no floors:
 // computeMaxSpeed, src_WENO/fluid.cpp:539-553.
 __global__ __launch_bounds__(256) void fv_max_speed_kernel(const double *__restrict__ U, size_t np,
@@ -83,6 +97,10 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     const int ux = H.g.nx - 6, uy = H.g.ny - 6;
     A.ring = H.ring;
     A.nring = H.nring;
+    A.bad = H.cfg.positivity ? H.bad : nullptr;
+    A.any = H.cfg.positivity ? H.any + stage : nullptr;
+    A.blist = H.blist;
+    A.inst = 0;
     const int nrb = (H.nring + FV_THREADS - 1) / FV_THREADS;
     const bool f32 = H.cfg.precision == ZK_PREC_FP32_BULK;
     // tile shape: 0 = automatic (16x15 for small grids, 32x15 otherwise), 1 = 16x15, 2 = 32x15
@@ -103,6 +121,12 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
         const dim3 grid(A.ntiles + nrb);
         if (f32) hipLaunchKernelGGL((fv_stage_kernel<32, 15, true>), grid, dim3(FV_THREADS), 0, H.stream, A);
         else hipLaunchKernelGGL((fv_stage_kernel<32, 15, false>), grid, dim3(FV_THREADS), 0, H.stream, A);
+    }
+    if (A.bad && tiles) {
+        ProfScope pf(H, "fv_fallback");
+        const dim3 grid(64);  // grid-strided over the (few) bad cells; ends at once when there are none
+        if (f32) hipLaunchKernelGGL(fv_fallback_kernel<true>, grid, dim3(256), 0, H.stream, A, H.any, stage);
+        else hipLaunchKernelGGL(fv_fallback_kernel<false>, grid, dim3(256), 0, H.stream, A, H.any, stage);
     }
 }
 

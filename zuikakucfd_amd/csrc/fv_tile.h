@@ -35,7 +35,21 @@ struct FvArgs {
     int ntiles;          // blocks past ntiles combine the ghost ring instead
     const int32_t *ring; // cells outside the update range
     int nring;
+    uint8_t *bad;        // cell positivity fallback (positivity on): flags of the stage, else null
+    unsigned int *any;   // count of bad cells of this stage
+    int2 *blist;         // the bad cells of the stage: (instance, cell)
+    int inst;            // instance index written to blist (0 for a single grid)
 };
+
+// A stage result is bad when rho or p is not above 1e-12 (or not finite):
+// rho > 1e-12 and 2 rho E - |m|^2 > 2 rho 1e-12 / (gamma - 1), which is
+// p > 1e-12 multiplied through by 2 rho / (gamma - 1) > 0.
+__device__ __forceinline__ bool fv_bad_state(const double U[4], double gamma)
+{
+    const double rho = U[0];
+    const double q = 2.0 * rho * U[3] - (U[1] * U[1] + U[2] * U[2]);
+    return !(rho > 1e-12 && q > 2.0 * rho * 1e-12 / (gamma - 1.0));
+}
 
 // Face positivity check in FP64 (an option with no reference counterpart).
 __device__ __forceinline__ bool face_positive(const double Q[4], double gamma)
@@ -43,6 +57,68 @@ __device__ __forceinline__ bool face_positive(const double Q[4], double gamma)
     if (!(Q[0] >= 1e-12)) return false;
     const double p = (gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
     return p >= 1e-12;
+}
+
+// Numerical flux of one face from the 6 cells across it (get(v, k), k = 0..5,
+// the face lies between k = 2 and k = 3): WENO5 reconstruction of both face
+// states (FP64, or FP32 on deltas to the face's left cell), the optional face
+// positivity check, HLLC/HLL; y faces call the flux with components 1 and 2
+// swapped and swap the result back (fluid.cpp:444-451).  F is returned as
+// (F0, F_rho_u, F_rho_v, F3).
+template <bool F32, bool X, typename Get>
+__device__ __forceinline__ void fv_face_flux(const FvArgs &A, Get get, double F[4])
+{
+    double QL[4], QR[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double w[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = get(v, k);
+        if (F32) {
+            const double rf = w[2];
+            float dl, drr;
+            weno5_pair_f32((float)(w[0] - rf), (float)(w[1] - rf), 0.0f, (float)(w[3] - rf), (float)(w[4] - rf),
+                           (float)(w[5] - rf), dl, drr);
+            QL[v] = rf + (double)dl;
+            QR[v] = rf + (double)drr;
+        } else {
+            weno5_pair_f64(w[0], w[1], w[2], w[3], w[4], w[5], QL[v], QR[v]);
+        }
+    }
+    if (A.positivity && !(face_positive(QL, A.gamma) && face_positive(QR, A.gamma))) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            QL[v] = get(v, 2);
+            QR[v] = get(v, 3);
+        }
+    }
+    double L[4], R[4], G[4];
+    L[0] = QL[0]; R[0] = QR[0]; L[3] = QL[3]; R[3] = QR[3];
+    L[1] = X ? QL[1] : QL[2]; L[2] = X ? QL[2] : QL[1];
+    R[1] = X ? QR[1] : QR[2]; R[2] = X ? QR[2] : QR[1];
+    if (A.flux == ZK_FLUX_HLL) hll_f64(L, R, A.gamma, G);
+    else if (F32) hllc_fv_f32(L, R, A.gamma, G);
+    else hllc_fv_fast_f64(L, R, A.gamma, G);
+    F[0] = G[0];
+    F[1] = X ? G[1] : G[2];
+    F[2] = X ? G[2] : G[1];
+    F[3] = G[3];
+}
+
+// First-order flux of a face (the two cell values, FP64), as fv_face_flux.
+template <bool X>
+__device__ __forceinline__ void fv_face_flux_fo(const FvArgs &A, const double a[4], const double b[4], double F[4])
+{
+    double L[4], R[4], G[4];
+    L[0] = a[0]; R[0] = b[0]; L[3] = a[3]; R[3] = b[3];
+    L[1] = X ? a[1] : a[2]; L[2] = X ? a[2] : a[1];
+    R[1] = X ? b[1] : b[2]; R[2] = X ? b[2] : b[1];
+    if (A.flux == ZK_FLUX_HLL) hll_f64(L, R, A.gamma, G);
+    else hllc_fv_fast_f64(L, R, A.gamma, G);
+    F[0] = G[0];
+    F[1] = X ? G[1] : G[2];
+    F[2] = X ? G[2] : G[1];
+    F[3] = G[3];
 }
 
 // Face sweep of one direction over the tile: every face computed once.
@@ -66,44 +142,10 @@ __device__ __forceinline__ void fv_sweep(const FvArgs &A, double (*s_u)[TY + 6][
             if (j0 + c >= A.nx - 3 || i0 + r - 1 >= A.ny - 3) continue;
             sr = r; sc = c + 3;
         }
-        double QL[4], QR[4];
+        double F[4];
+        fv_face_flux<F32, X>(A, [&](int v, int k) { return s_u[v][sr + k * dr][sc + k * dc]; }, F);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double w[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) w[k] = s_u[v][sr + k * dr][sc + k * dc];
-            if (F32) {
-                // FP32 reconstruction of the deltas to the face's left cell
-                const double rf = w[2];
-                float dl, drr;
-                weno5_pair_f32((float)(w[0] - rf), (float)(w[1] - rf), 0.0f, (float)(w[3] - rf), (float)(w[4] - rf),
-                               (float)(w[5] - rf), dl, drr);
-                QL[v] = rf + (double)dl;
-                QR[v] = rf + (double)drr;
-            } else {
-                weno5_pair_f64(w[0], w[1], w[2], w[3], w[4], w[5], QL[v], QR[v]);
-            }
-        }
-        if (A.positivity && !(face_positive(QL, A.gamma) && face_positive(QR, A.gamma))) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                QL[v] = s_u[v][sr + 2 * dr][sc + 2 * dc];
-                QR[v] = s_u[v][sr + 3 * dr][sc + 3 * dc];
-            }
-        }
-        // y faces: normal momentum is rho*v (component 2); the flux is called
-        // with components 1 and 2 swapped and swapped back (fluid.cpp:444-451).
-        double L[4], R[4], F[4];
-        L[0] = QL[0]; R[0] = QR[0]; L[3] = QL[3]; R[3] = QR[3];
-        L[1] = X ? QL[1] : QL[2]; L[2] = X ? QL[2] : QL[1];
-        R[1] = X ? QR[1] : QR[2]; R[2] = X ? QR[2] : QR[1];
-        if (A.flux == ZK_FLUX_HLL) hll_f64(L, R, A.gamma, F);
-        else if (F32) hllc_fv_f32(L, R, A.gamma, F);
-        else hllc_fv_fast_f64(L, R, A.gamma, F);
-        s_f[0][f] = F[0];
-        s_f[1][f] = X ? F[1] : F[2];
-        s_f[2][f] = X ? F[2] : F[1];
-        s_f[3][f] = F[3];
+        for (int v = 0; v < 4; ++v) s_f[v][f] = F[v];
     }
 }
 
@@ -164,15 +206,144 @@ __device__ __forceinline__ void fv_tile_body(const FvArgs &A, int tile, double (
         if (i >= ny - 3 || j >= nx - 3) continue;
         const size_t k = (size_t)i * nx + j;
         const bool upd = A.upd[k] != 0;
+        double out[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             double dU = 0.0;
             if (upd) dU = -(xs[q][v] + dtdy * (s_f[v][(r + 1) * TX + c] - s_f[v][r * TX + c]));
             const double un = A.Un[v * np + k];
-            A.Uout[v * np + k] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (s_u[v][r + 3][c + 3] + dU);
+            out[v] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (s_u[v][r + 3][c + 3] + dU);
         }
+        if (A.bad) {
+            // cell positivity fallback, pass 1: a bad cell is left unwritten
+            // (the stage may update in place) and recomputed by fv_fallback_cell
+            const bool bad = upd && fv_bad_state(out, A.gamma);
+            A.bad[k] = (uint8_t)bad;
+            if (bad) {
+                const unsigned int slot = atomicAdd(A.any, 1u);
+                A.blist[slot] = int2{A.inst, (int)k};
+                continue;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) A.Uout[v * np + k] = out[v];
     }
 }
 
+// Pass 2 driver for one bad cell k: the bad cell itself, then each good
+// updated neighbour whose lowest-index bad neighbour is k (so that every
+// affected cell is handled by exactly one thread, in a fixed way).
+template <bool F32>
+__device__ __forceinline__ void fv_fallback_bad(const FvArgs &A, size_t k);
 
+// Cell positivity fallback, pass 2, for updated cell (i, j): a bad cell is
+// recomputed with first-order fluxes on its four faces; a good cell next to
+// bad cells replaces the high-order flux of each shared face by the
+// first-order one (the high-order flux recomputed exactly as pass 1 did).
+// Every face touching a bad cell is then first order on both of its sides.
+template <bool F32>
+__device__ __forceinline__ void fv_fallback_cell(const FvArgs &A, int i, int j)
+{
+    const int nx = A.nx;
+    const size_t np = A.np, k = (size_t)i * nx + j;
+    if (!A.upd[k]) return;
+    const bool b0 = A.bad[k], bE = A.bad[k + 1], bW = A.bad[k - 1], bS = A.bad[k + nx], bN = A.bad[k - nx];
+    if (!(b0 || bE || bW || bS || bN)) return;
+    auto U = [&](size_t kk, int v) {
+        const int32_t e = A.src[kk];
+        double x = A.Uin[v * np + (size_t)(e >> kSrcShift)];
+        if ((v == 1 && (e & 1)) || (v == 2 && (e & 2))) x = -x;
+        return x;
+    };
+    auto cell = [&](size_t kk, double c[4]) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = U(kk, v);
+    };
+    const double dt = A.S->dt, dtdx = dt / A.da, dtdy = dt / A.da;
+    // face (a | b) along x at row i, columns jl | jl+1, or along y at column j, rows il | il+1
+    auto fo_x = [&](int jl, double F[4]) {
+        double a[4], b[4];
+        cell((size_t)i * nx + jl, a);
+        cell((size_t)i * nx + jl + 1, b);
+        fv_face_flux_fo<true>(A, a, b, F);
+    };
+    auto fo_y = [&](int il, double F[4]) {
+        double a[4], b[4];
+        cell((size_t)il * nx + j, a);
+        cell((size_t)(il + 1) * nx + j, b);
+        fv_face_flux_fo<false>(A, a, b, F);
+    };
+    auto hi_x = [&](int jl, double F[4]) {
+        fv_face_flux<F32, true>(A, [&](int v, int t) { return U((size_t)i * nx + (jl - 2 + t), v); }, F);
+    };
+    auto hi_y = [&](int il, double F[4]) {
+        fv_face_flux<F32, false>(A, [&](int v, int t) { return U((size_t)(il - 2 + t) * nx + j, v); }, F);
+    };
+    if (b0) {
+        double FE[4], FW[4], GS[4], GN[4];
+        fo_x(j, FE);
+        fo_x(j - 1, FW);
+        fo_y(i, GS);
+        fo_y(i - 1, GN);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double dU = -(dtdx * (FE[v] - FW[v]) + dtdy * (GS[v] - GN[v]));
+            const double un = A.Un[v * np + k];
+            A.Uout[v * np + k] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (U(k, v) + dU);
+        }
+        return;
+    }
+    double d[4] = {0.0, 0.0, 0.0, 0.0}, lo[4], hi[4];
+    if (bE) {
+        fo_x(j, lo); hi_x(j, hi);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) d[v] -= dtdx * (lo[v] - hi[v]);
+    }
+    if (bW) {
+        fo_x(j - 1, lo); hi_x(j - 1, hi);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) d[v] += dtdx * (lo[v] - hi[v]);
+    }
+    if (bS) {
+        fo_y(i, lo); hi_y(i, hi);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) d[v] -= dtdy * (lo[v] - hi[v]);
+    }
+    if (bN) {
+        fo_y(i - 1, lo); hi_y(i - 1, hi);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) d[v] += dtdy * (lo[v] - hi[v]);
+    }
+    const double w = (A.stage == 0) ? 1.0 : A.cb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) A.Uout[v * np + k] += w * d[v];
+}
+
+
+}  // namespace zk
+
+namespace zk {
+template <bool F32>
+__device__ __forceinline__ void fv_fallback_bad(const FvArgs &A, size_t k)
+{
+    const int nx = A.nx;
+    const int i = (int)(k / nx), j = (int)(k % nx);
+    fv_fallback_cell<F32>(A, i, j);
+    const long off[4] = {-(long)nx, -1, 1, (long)nx};
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const size_t n = (size_t)((long)k + off[d]);
+        const int ni = (int)(n / nx), nj = (int)(n % nx);
+        if (ni < 3 || ni >= A.ny - 3 || nj < 3 || nj >= nx - 3) continue;  // outside the update range
+        if (A.bad[n] || !A.upd[n]) continue;
+        // lowest-index bad neighbour of n
+        size_t first = k;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const size_t m = (size_t)((long)n + off[e]);
+            if (A.bad[m] && m < first) first = m;
+        }
+        if (first == k) fv_fallback_cell<F32>(A, ni, nj);
+    }
+}
 }  // namespace zk

@@ -92,14 +92,14 @@ void free_all(Handle *H)
 {
     drop_graph(H);
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
-                    H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand};
+                    H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand, H->bad, H->any, H->blist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
     H->src = nullptr; H->upd = nullptr; H->ring = nullptr;
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
     H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
-    H->stats = nullptr; H->score = nullptr; H->cand = nullptr;
+    H->stats = nullptr; H->score = nullptr; H->cand = nullptr; H->bad = nullptr; H->any = nullptr; H->blist = nullptr;
     H->loaded = false;
 }
 
@@ -313,6 +313,13 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     ZK_HIP(hipMalloc(&H->flags, np));
     ZK_HIP(hipMalloc(&H->scal, sizeof(zk::StepScalars)));
     ZK_HIP(hipMalloc(&H->lam_bits, sizeof(unsigned long long)));
+    if (H->cfg.method == ZK_METHOD_WENO && H->cfg.positivity) {
+        ZK_HIP(hipMalloc(&H->bad, np));
+        ZK_HIP(hipMemset(H->bad, 0, np));
+        ZK_HIP(hipMalloc(&H->any, 3 * sizeof(unsigned int)));
+        ZK_HIP(hipMemset(H->any, 0, 3 * sizeof(unsigned int)));
+        ZK_HIP(hipMalloc(&H->blist, np * sizeof(int2)));
+    }
     if (H->cfg.method == ZK_METHOD_DG) {
         const int NQ = H->cfg.P + 1;
         const size_t nxu = nx - 6, nyu = ny - 6;

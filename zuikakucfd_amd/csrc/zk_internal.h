@@ -77,6 +77,9 @@ struct Handle {
     unsigned long long *stats = nullptr;  // audited, agree, miss, false flag, mlp flagged, band, candidates, audited TVB positives
     float *score = nullptr;
     int32_t *cand = nullptr;  // [0] = count, then the candidate cells of the stage
+    uint8_t *bad = nullptr;       // WENO cell positivity fallback: bad-cell flags of the stage
+    unsigned int *any = nullptr;  // bad-cell counts, one per stage
+    int2 *blist = nullptr;        // the bad cells of the current stage
     int ncu = 0;              // compute units of the device
 
     // kernel timing: when prof is set every launcher brackets its kernel with
