@@ -298,6 +298,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the other configurations")
     ap.add_argument("--path-steps", type=int, default=20)
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="steps of every instance in the batched-ensemble configuration (200 as specified)")
     ap.add_argument("--learned-problems", type=int, default=4,
                     help="seeded problems of the DG P=2 learned-indicator configuration (of 64)")
     ap.add_argument("--profile-steps", type=int, default=3)
@@ -367,7 +369,7 @@ def main() -> int:
     if not args.no_paths:
         line["paths"] = {"weno_fp64_riemann256": path_weno_riemann256(4 * args.path_steps),
                          "jet_fp32_bulk": path_jet(args.path_steps),
-                         "ensemble256": path_ensemble(args.path_steps),
+                         "ensemble256": path_ensemble(args.ensemble_steps),
                          "dg2_learned2048": path_learned(args.path_steps, args.learned_problems)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, m, dt)

@@ -170,12 +170,13 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
 
 /* Kernel-variant selection for the autotuner (no reference counterpart):
  * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
- * automatic (fused row-marching stage kernel at P = 3 without the Persson
- * limiter, split face/cell kernels otherwise), 10 + R = fused kernel with
- * R-row segments (R in 4..64, any P), 2/3/4 = split kernels with the P=3
- * MFMA cell kernel at that minimum of waves per SIMD, -1 = split kernels
- * with the VALU cell kernel; use_graph = 1 (default) lets zk_step replay
- * a captured WENO step as a HIP graph. */
+ * automatic (fused row-marching stage kernel, workgroup form, at P >= 2
+ * without the Persson limiter; split face/cell kernels otherwise), 100 + R =
+ * fused workgroup form with R-row segments (R in 4..64; 100 = R chosen to
+ * fill whole rounds of the device), 10 + R = fused single-wave form,
+ * 2/3/4 = split kernels with the P=3 MFMA cell kernel at that minimum of
+ * waves per SIMD, -1 = split kernels with the VALU cell kernel; use_graph = 1
+ * (default) lets zk_step replay a captured WENO step as a HIP graph. */
 zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph);
 
 /* ---------------------------------------------------------------------------

@@ -164,15 +164,16 @@ def test_dg_tvb_riemann(P):
 
 
 @pytest.mark.parametrize("P", [1, 2, 3])
-@pytest.mark.parametrize("variant", [0, 14, 2])
-def test_dg_random_mask(P, variant):
+@pytest.mark.parametrize("variant,nx", [(0, 53), (14, 53), (104, 53), (2, 53), (104, 150), (100, 150)])
+def test_dg_random_mask(P, variant, nx):
     """Irregular masks (interior -1 islands, slip cells) through every stage
     kernel variant: automatic, the fused row-marching kernel with 4-row
-    segments (segment and run edges everywhere, a partial last run), and the
-    split face/cell kernels.  TVB flags must match too."""
-    say(f"dg P={P} random mask variant={variant}")
+    segments (segment and run edges everywhere, a partial last run), its
+    workgroup form (one partial 62-column run at nx = 53, three runs at
+    nx = 150), and the split face/cell kernels.  TVB flags must match too."""
+    say(f"dg P={P} random mask variant={variant} nx={nx}")
     rng = np.random.default_rng(11 + P)
-    ny, nx = 37, 53
+    ny = 37
     m = meshio.riemann_2d(64)
     m.rho = 0.5 + rng.random((ny, nx)); m.u = rng.uniform(-1, 1, (ny, nx))
     m.v = rng.uniform(-1, 1, (ny, nx)); m.p = 0.5 + rng.random((ny, nx))
@@ -181,6 +182,9 @@ def test_dg_random_mask(P, variant):
     b[25, 5:9] = -1
     b[5:7, 40] = 1
     b[30, 30:33] = 1
+    if nx > 100:
+        b[20:24, 70] = -1
+        b[8, 120:125] = 1
     m.bctype = b
     m.nx, m.ny = nx, ny
     s, o = dg_pair(m, P, zk.LIMITER_TVB, M=50.0)

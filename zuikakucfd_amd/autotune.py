@@ -45,8 +45,8 @@ def solver_candidates(method: int, P: int) -> list[tuple[int, int]]:
     if method == zk.METHOD_WENO:
         return [(1, 0), (2, 0)]
     if P == 3:
-        return [(0, 0), (0, 18), (0, 26), (0, 2), (0, 3)]
-    return [(0, 0), (0, 26), (0, -1)]
+        return [(0, 0), (0, 117), (0, 134), (0, 26), (0, 2)]
+    return [(0, 0), (0, 117), (0, -1)]
 
 
 def tune_solver(factory: Callable[[], zk.Solver], dt: float, key: str, steps: int = 4,

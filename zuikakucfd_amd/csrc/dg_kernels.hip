@@ -840,22 +840,42 @@ MfmaTabsF make_mfma_tables_fused(int P)
     return M;
 }
 
-template <int P, int MINW>
+// WG = false: every wave is independent and owns 16 columns, 14 of them
+// updated.  WG = true: the four waves of a workgroup march the same rows side
+// by side over 64 columns, 62 of them updated; the x faces between waves take
+// the neighbour wave's boundary trace and face projection from LDS (two
+// barriers per row), so halo columns exist only at workgroup edges.
+template <int P, int MINW, bool WG>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const MfmaTabsF &M = c_mff[P - 1];
     const DgTables &T = c_tab[P - 1];
+    __shared__ double lds_w[WG ? 4 : 1][4][4];   // [wave][g][c]: W trace of the wave's column 0
+    __shared__ double lds_ge[WG ? 4 : 1][4][4];  // [wave][g][c]: projected flux of its column-15 face
     if (A.S->done) return;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wv = threadIdx.x >> 6;
     const int nseg = (A.nyu + R - 1) / R;
-    if (w >= nrun * nseg) return;  // whole wave
-    const int run = w % nrun, seg = w / nrun;
-    const int j = 2 + run * 14 + col;                 // column of this lane
+    int run, seg, j;
+    bool store_col;
+    if (WG) {
+        if ((int)blockIdx.x >= nrun * nseg) return;  // whole workgroup
+        run = blockIdx.x % nrun;
+        seg = blockIdx.x / nrun;
+        const int x = 16 * wv + col;
+        j = 2 + run * 62 + x;
+        store_col = x >= 1 && x <= 62 && j <= A.nx - 4;
+    } else {
+        const int w = blockIdx.x * (blockDim.x >> 6) + wv;
+        if (w >= nrun * nseg) return;  // whole wave
+        run = w % nrun;
+        seg = w / nrun;
+        j = 2 + run * 14 + col;
+        store_col = col >= 1 && col <= 14 && j <= A.nx - 4;
+    }
     const int jc = j < A.nx ? j : A.nx - 1;           // clamped for loads past the right edge
-    const bool store_col = col >= 1 && col <= 14 && j <= A.nx - 4;
     const int r0 = 3 + seg * R;
     const int r1 = (r0 + R < A.ny - 3) ? r0 + R : A.ny - 3;  // rows [r0, r1), halo rows r0-1 and r1
     const size_t np = A.np;
@@ -982,16 +1002,37 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 Rx[c] = __shfl_down(Tr[c][0], 1, 16);
                 Sup[c] = Tr[c][3];
             }
+            if (WG) {
+                if (col == 0)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) lds_w[wv][g][c] = Tr[c][0];
+                __syncthreads();
+                if (col == 15 && wv < 3)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) Rx[c] = lds_w[wv + 1][g][c];
+            }
             if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
             else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
+            double gE[4], gW[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const d4 ge = __builtin_amdgcn_mfma_f64_16x16x4f64(M.axA[ln], Fx[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
-                const double gE = ge[0];                   // py = g projection of face (col | col+1)
-                const double gW = __shfl_up(gE, 1, 16);    // face (col-1 | col)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[c][v] += T.sgn[v < NQ ? v : 0] * gW - gE;
+                gE[c] = ge[0];                             // py = g projection of face (col | col+1)
+                gW[c] = __shfl_up(gE[c], 1, 16);           // face (col-1 | col)
             }
+            if (WG) {
+                if (col == 15)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) lds_ge[wv][g][c] = gE[c];
+                __syncthreads();
+                if (col == 0 && wv > 0)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wv - 1][g][c];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[c][v] += T.sgn[v < NQ ? v : 0] * gW[c] - gE[c];
         }
         {
             d4 U[4];
@@ -1430,25 +1471,36 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     const size_t ncell = (size_t)A.nxu * A.nyu;
     // variant: 0 auto (fused), -1 split with the VALU cell kernel, 2..4 split with the
     // P=3 MFMA cell kernel at that occupancy, 10+R fused with R-row segments
-    // auto picks the fused kernel for P = 3; at P <= 2 its padded 4x4 slots waste
-    // more than the split kernels' extra traffic costs (measured on 1024^2)
+    // auto picks the fused kernel (workgroup form) for P >= 2; at P = 1 its padded
+    // 4x4 slots waste three quarters of the work, and the split kernels run
     const bool fused = H.cfg.limiter != ZK_LIMITER_PERSSON_ZS && !H.no_mfma &&
-                       ((H.tune == 0 && P == 3) || H.tune >= 10);
+                       ((H.tune == 0 && P >= 2) || H.tune >= 10);
     if (ncell > 0 && fused) {
-        int R = H.tune >= 10 ? H.tune - 10 : 0;
-        const int nrun = (A.nxu + 13) / 14;
+        // automatic: the workgroup form; 10+R / 100+R force a form and segment length
+        const bool wg = H.tune == 0 || H.tune >= 100;
+        int R = H.tune >= 100 ? H.tune - 100 : (H.tune >= 10 ? H.tune - 10 : 0);
+        const int width = wg ? 62 : 14;                 // updated columns per wave (per workgroup for wg)
+        const int nrun = (A.nxu + width - 1) / width;
         if (R <= 0) {
-            // about four segment waves per SIMD over the device, rows in [8, 64]
-            const int simds = 4 * (H.ncu > 0 ? H.ncu : 256);
-            const long target = 4L * simds;
-            const long nseg = (target + nrun - 1) / nrun;
-            R = (int)((A.nyu + nseg - 1) / nseg);
-            R = R < 8 ? 8 : (R > 64 ? 64 : R);
+            // Whole rounds of the device: at 2 waves per SIMD a CU holds 2
+            // workgroups (or 8 single waves).  The fewest rounds with R <= 64
+            // keep halo rows few and leave no part-filled last round.
+            const long cap = (long)(H.ncu > 0 ? H.ncu : 256) * (wg ? 2 : 8);
+            for (long m = 1;; ++m) {
+                const long nseg = m * cap / nrun;
+                if (nseg < 1) continue;
+                R = (int)((A.nyu + nseg - 1) / nseg);
+                if (R <= 64 || nseg >= A.nyu) break;
+            }
+            R = R < 4 ? 4 : R;
         }
-        const long waves = (long)nrun * ((A.nyu + R - 1) / R);
+        const long units = (long)nrun * ((A.nyu + R - 1) / R);
         ProfScope ps(H, "dg_fused");
-        hipLaunchKernelGGL((dg_fused_kernel<P, 2>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, H.stream, A, R,
-                           nrun);
+        if (wg)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R, nrun);
+        else
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
+                               H.stream, A, R, nrun);
     } else if (ncell > 0) {
         {
             ProfScope ps(H, "dg_face_x");

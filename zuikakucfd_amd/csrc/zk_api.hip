@@ -616,9 +616,10 @@ zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph
     Handle *H = reinterpret_cast<Handle *>(h);
     if (!H) return ZK_ERR_ARG;
     const bool dg_ok = dg_variant == -1 || dg_variant == 0 || (dg_variant >= 2 && dg_variant <= 4) ||
-                       (dg_variant >= 14 && dg_variant <= 74);
+                       (dg_variant >= 14 && dg_variant <= 74) || dg_variant == 100 ||
+                       (dg_variant >= 104 && dg_variant <= 164);
     if (fv_tile < 0 || fv_tile > 2 || !dg_ok) {
-        H->err = "zk_set_tuning: fv_tile in 0..2, dg_variant -1, 0, 2..4 or 10+R with R in 4..64";
+        H->err = "zk_set_tuning: fv_tile in 0..2, dg_variant -1, 0, 2..4, 10+R or 100+R with R in 4..64 (100: automatic R)";
         return ZK_ERR_ARG;
     }
     drop_graph(H);
