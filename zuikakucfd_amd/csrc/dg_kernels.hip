@@ -177,6 +177,7 @@ struct DgArgs {
     const StepScalars *S;
     double *gx, *gy;
     double *means;    // stage means (4 planes) for the TVB limiter, or null
+    uint8_t *zsok;    // fused kernel: 1 where the positivity scaling is a no-op (TVB limiters), or null
     int ny, nx, nxu, nyu;
     size_t np, nfx, nfy;
     double da, gamma;
@@ -786,6 +787,7 @@ struct MfmaTabsF {
     double axA[64];      // x-face projection: row i -> w_q L_{i%4}(s_q), column q = l/16
     double ayA[64];      // y-face projection: row i -> w_q L_{i/4}(s_q)
     double zsA[4][64];   // positivity points: truncated basis() at (a, b), point l%16 = 4a + b
+    double zc[16];       // slot 4px+py: max over the 16 positivity points of |truncated basis()|
 };
 __constant__ MfmaTabsF c_mff[3];
 
@@ -836,6 +838,14 @@ MfmaTabsF make_mfma_tables_fused(int P)
         for (int l = 0; l < 64; ++l) {
             const int i = l % 16, a = i / 4, b = i % 4, px = s, py = l / 16;
             M.zsA[s][l] = (px <= P && py <= P) ? zsP_h(px, zsp[a]) * zsP_h(py, zsp[b]) : 0.0;
+        }
+    for (int px = 0; px < 4; ++px)
+        for (int py = 0; py < 4; ++py) {
+            double mx = 0.0;
+            if (px <= P && py <= P)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b) mx = std::max(mx, std::fabs(zsP_h(px, zsp[a]) * zsP_h(py, zsp[b])));
+            M.zc[4 * px + py] = mx;
         }
     return M;
 }
@@ -966,6 +976,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             for (int c = 0; c < 4; ++c)
                 gN[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(M.ayA[ln], G[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
         }
+        // sab[c]: sum over the non-mean modes of |mode| * max |basis| at the positivity points
+        double sab[4] = {0.0, 0.0, 0.0, 0.0}, mean0[4] = {0.0, 0.0, 0.0, 0.0};
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
@@ -974,8 +986,36 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 for (int v = 0; v <= P; ++v) {
                     const double out = base[c][v] - fs[v] * gN[c][v];
                     (A.Qout + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)kp] = out;
-                    if (A.means && v == 0 && g == 0) A.means[(size_t)c * np + kp] = out;
+                    if (v == 0 && g == 0) {
+                        mean0[c] = out;
+                        if (A.means) A.means[(size_t)c * np + kp] = out;
+                    } else {
+                        sab[c] += fabs(out) * M.zc[4 * v + g];
+                    }
                 }
+        }
+        if (A.zsok && r > r0) {
+            // Sufficient condition for the positivity scaling to leave the cell
+            // alone: lower bounds of rho and p over its 16 points (means minus
+            // the mode sums) clear the 1e-5 floors with a margin.
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                sab[c] += __shfl_xor(sab[c], 16);
+                sab[c] += __shfl_xor(sab[c], 32);
+                mean0[c] = __shfl(mean0[c], col);
+            }
+            if (g == 0 && store_col) {
+                const double rlo = mean0[0] - sab[0];
+                const double elo = mean0[3] - sab[3];
+                const double mx = fabs(mean0[1]) + sab[1], my = fabs(mean0[2]) + sab[2];
+                const double tol = 1e-12 * (fabs(mean0[0]) + sab[0] + fabs(mean0[3]) + sab[3] + mx + my);
+                bool ok = rlo > 2e-5 + tol;
+                if (ok) {
+                    const double plo = (gam - 1.0) * (elo - 0.5 * (mx * mx + my * my) / rlo);
+                    ok = plo > 2e-5 + tol;
+                }
+                A.zsok[kp] = (uint8_t)ok;
+            }
         }
         if (!inner) break;
         // U^n of row r, needed at the end of this iteration
@@ -1082,11 +1122,17 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 // scaling evaluates rho and then all variables at the 4x4 points with MFMA
 // products and takes the minimum theta across the column's lanes.  Same
 // results as dg_tvb_kernel up to the summation order of the point values.
+//
+// With zsok (written by the fused stage kernel) the kernel first loads only
+// what the TVB test reads (means and linear modes); a wave whose cells are
+// all unlimited and certified positive by zsok ends there, and only the other
+// waves load every mode for the limiting and the positivity scaling.
 template <int P>
 __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
                                                           const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                           int ny, int nx, size_t np, double Mh2, double gamma,
+                                                          const uint8_t *__restrict__ zsok,
                                                           const StepScalars *__restrict__ S)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
@@ -1103,10 +1149,14 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     const bool slot = g <= P;
     const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
     double C[4][4];
+    // phase 1: the modes the TVB test reads, (0,0), (1,0) in group 0 and (0,1) in group 1
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) C[c][v] = (v <= P && slot) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+        for (int v = 0; v < 4; ++v) {
+            const bool need = (g == 0 && v <= 1) || (g == 1 && v == 0);
+            C[c][v] = need ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+        }
 
     // ---- TVB (cells with bctype 0) ----
     int chg = 0;  // bit c: variable c limited
@@ -1128,6 +1178,22 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
         ayl[c] = __shfl(a_y, col);
     }
     chg = __shfl(chg, col);
+    {
+        const bool sure = zsok && zsok[k];  // positivity scaling certified to be a no-op
+        const bool need = live && (chg != 0 || !sure);
+        if (!__any(need)) {
+            if (live && g == 0) flags[k] = 0;
+            return;
+        }
+    }
+    // phase 2: every mode
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const bool had = (g == 0 && v <= 1) || (g == 1 && v == 0);
+            if (!had && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+        }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
         if (chg & (1 << c))
@@ -1458,6 +1524,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.src = H.src; A.upd = H.upd; A.bct = H.bct; A.S = H.scal;
     A.gx = H.gx; A.gy = H.gy;
     A.means = (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) ? H.means : nullptr;
+    A.zsok = nullptr;  // set below when the fused kernel runs under the TVB limiter
     A.ny = H.g.ny; A.nx = H.g.nx; A.nxu = H.g.nx - 6; A.nyu = H.g.ny - 6;
     A.np = H.g.np;
     A.nfx = (size_t)(A.nxu + 1) * A.nyu;
@@ -1495,6 +1562,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             R = R < 4 ? 4 : R;
         }
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
+        if (H.cfg.limiter == ZK_LIMITER_TVB) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
         if (wg)
             hipLaunchKernelGGL((dg_fused_kernel<P, 2, true>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R, nrun);
@@ -1540,12 +1608,15 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB) {
         ProfScope ps(H, "dg_tvb");
-        if (H.no_mfma || P < 3)  // the per-thread kernel is faster at P <= 2 (1024^2)
+        // the matrix-core kernel when the fused stage certified positivity (any P) or
+        // at P = 3; the per-thread kernel otherwise (faster at P <= 2 without the screen)
+        if (H.no_mfma || (P < 3 && !A.zsok))
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
                                H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
         else
             hipLaunchKernelGGL((dg_tvb_mfma_kernel<P>), dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, H.stream,
-                               Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
+                               Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, A.zsok,
+                               H.scal);
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {

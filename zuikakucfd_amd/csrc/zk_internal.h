@@ -80,6 +80,7 @@ struct Handle {
     uint8_t *bad = nullptr;       // WENO cell positivity fallback: bad-cell flags of the stage
     unsigned int *any = nullptr;  // bad-cell counts, one per stage
     int2 *blist = nullptr;        // the bad cells of the current stage
+    uint8_t *zsok = nullptr;      // DG TVB: cells whose positivity scaling is certified a no-op
     int ncu = 0;              // compute units of the device
 
     // kernel timing: when prof is set every launcher brackets its kernel with
