@@ -1114,6 +1114,31 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     }
 }
 
+// The TVB test of one cell (per variable: modified minmod of the two linear
+// modes against the filled neighbour means); returns whether any variable
+// changes and leaves the limited slopes in axl/ayl.
+template <int P>
+__device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], const double *__restrict__ M,
+                                         const int32_t *__restrict__ src, size_t np, size_t k, int nx, double Mh2,
+                                         double axl[4], double ayl[4], bool chg[4])
+{
+    constexpr int NQ = P + 1;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double ub = C[c][0], ax = C[c][NQ], ay = C[c][1];
+        axl[c] = ax;
+        ayl[c] = ay;
+        if (!(fabs(ax) <= Mh2))
+            axl[c] = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
+        if (!(fabs(ay) <= Mh2))
+            ayl[c] = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
+        chg[c] = (axl[c] != ax || ayl[c] != ay);
+        any = any || chg[c];
+    }
+    return any;
+}
+
 // TVB limiter then positivity scaling in the matrix-core layout of the fused
 // kernel: a wave owns 16 consecutive cells of the update range, lane (g, col)
 // holds modes (px = v, py = g) of cell col.  The TVB test of a cell runs in its
@@ -1127,25 +1152,18 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 // what the TVB test reads (means and linear modes); a wave whose cells are
 // all unlimited and certified positive by zsok ends there, and only the other
 // waves load every mode for the limiting and the positivity scaling.
+// The TVB + positivity work of the 16 cells of one wave (cell k of column
+// lane col; live is false on padding lanes, which compute on a valid cell and
+// store nothing).  The whole wave must call it.
 template <int P>
-__global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
-                                                          const int32_t *__restrict__ src,
-                                                          const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
-                                                          int ny, int nx, size_t np, double Mh2, double gamma,
-                                                          const uint8_t *__restrict__ zsok,
-                                                          const StepScalars *__restrict__ S)
+__device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const double *__restrict__ Mn,
+                                               const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                               uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
+                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const MfmaTabsF &MT = c_mff[P - 1];
-    if (S->done) return;
-    const int nxu = nx - 6, nyu = ny - 6;
-    const size_t ncell = (size_t)nxu * nyu;
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
-    if (t0 >= ncell) return;  // whole wave
-    const bool live = t0 + col < ncell;
-    const size_t t = live ? t0 + col : ncell - 1;
-    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
     const bool slot = g <= P;
     const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
     double C[4][4];
@@ -1285,6 +1303,92 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
             for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] = C[c][v];
 }
 
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+                                                          const int32_t *__restrict__ src,
+                                                          const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
+                                                          int ny, int nx, size_t np, double Mh2, double gamma,
+                                                          const uint8_t *__restrict__ zsok,
+                                                          const StepScalars *__restrict__ S)
+{
+    if (S->done) return;
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t ncell = (size_t)nxu * nyu;
+    const int col = threadIdx.x & 15;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (t0 >= ncell) return;  // whole wave
+    const bool live = t0 + col < ncell;
+    const size_t t = live ? t0 + col : ncell - 1;
+    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live);
+}
+
+// TVB screen, one thread per cell: the TVB test from the mean and linear
+// modes; a cell that is unlimited and certified by zsok gets flag 0 and is
+// done, the others are listed (list[0] = count) for dg_tvb_list_kernel.
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_screen_kernel(const double *__restrict__ Q, const double *__restrict__ Mn,
+                                                            const int32_t *__restrict__ src,
+                                                            const int32_t *__restrict__ bct,
+                                                            uint8_t *__restrict__ flags,
+                                                            const uint8_t *__restrict__ zsok, int nx, int nxu,
+                                                            size_t ncell, size_t np, double Mh2,
+                                                            int32_t *__restrict__ list,
+                                                            const StepScalars *__restrict__ S)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    if (S->done) return;  // uniform over the grid
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    size_t k = 0;
+    if (t < ncell) {
+        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+        bool chg_any = false;
+        if (bct[k] == 0) {
+            double C[4][NM];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                C[c][0] = Q[(size_t)(c * NM) * np + k];
+                C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
+                C[c][NQ] = Q[(size_t)(c * NM + NQ) * np + k];
+            }
+            double axl[4], ayl[4];
+            bool chg[4];
+            chg_any = tvb_test<P>(C, Mn, src, np, k, nx, Mh2, axl, ayl, chg);
+        }
+        need = chg_any || !zsok[k];
+        if (!need) flags[k] = 0;
+    }
+    const unsigned long long mask = __ballot(need);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(list, __popcll(mask));
+    base = __shfl(base, leader);
+    if (need) list[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
+}
+
+// Full TVB + positivity work on the listed cells, 16 per wave, persistent grid.
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_list_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+                                                          const int32_t *__restrict__ src,
+                                                          const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
+                                                          int nx, size_t np, double Mh2, double gamma,
+                                                          const int32_t *__restrict__ list,
+                                                          const StepScalars *__restrict__ S)
+{
+    if (S->done) return;
+    const int n = list[0];
+    const int col = threadIdx.x & 15;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16; base < n; base += waves * 16) {
+        const bool live = base + col < n;
+        const size_t k = (size_t)list[1 + (live ? base + col : n - 1)];
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, k, live);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Learned troubled-cell indicator (P = 2)
 // ---------------------------------------------------------------------------
@@ -1297,31 +1401,6 @@ __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c)
     h *= 0x846CA68Bu;
     h ^= h >> 16;
     return h;
-}
-
-// The TVB test of one cell (per variable: modified minmod of the two linear
-// modes against the filled neighbour means); returns whether any variable
-// changes and leaves the limited slopes in axl/ayl.
-template <int P>
-__device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], const double *__restrict__ M,
-                                         const int32_t *__restrict__ src, size_t np, size_t k, int nx, double Mh2,
-                                         double axl[4], double ayl[4], bool chg[4])
-{
-    constexpr int NQ = P + 1;
-    bool any = false;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const double ub = C[c][0], ax = C[c][NQ], ay = C[c][1];
-        axl[c] = ax;
-        ayl[c] = ay;
-        if (!(fabs(ax) <= Mh2))
-            axl[c] = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
-        if (!(fabs(ay) <= Mh2))
-            ayl[c] = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
-        chg[c] = (axl[c] != ax || ayl[c] != ay);
-        any = any || chg[c];
-    }
-    return any;
 }
 
 // A cell can only be changed by the TVB test when one of its linear modes is
@@ -1413,6 +1492,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
                                                          int ny, int nx, size_t np, double Mh2, double gamma, float lo,
                                                          float hi, uint32_t audit_thresh, uint32_t salt,
                                                          unsigned long long *__restrict__ stats,
+                                                         const uint8_t *__restrict__ zsok,
                                                          const StepScalars *__restrict__ S)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
@@ -1422,12 +1502,26 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
     const int i = 3 + (int)(t / nxu), j = 3 + (int)(t % nxu);
     const size_t k = (size_t)i * nx + j;
     double C[4][NM];
+    // the mean and linear modes first: a non-candidate cell whose positivity
+    // scaling is certified a no-op (zsok) is left as it is
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        C[c][0] = Q[(size_t)(c * NM) * np + k];
+        C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
+        C[c][NQ] = Q[(size_t)(c * NM + NQ) * np + k];
+    }
+    const bool cand = bct[k] == 0 && tvb_candidate<P>(C, Mh2);
+    if (!cand && zsok && zsok[k]) {
+        flags[k] = 0;
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+        for (int m = 0; m < NM; ++m)
+            if (m != 0 && m != 1 && m != NQ) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
     int flag = 0;
-    if (bct[k] == 0 && tvb_candidate<P>(C, Mh2)) {
+    if (cand) {
         const float sc = score[k];
         const bool audit = audit_thresh == 0xFFFFFFFFu || hash3((uint32_t)k, salt, 0x5bd1e995u) < audit_thresh;
         const bool band = sc >= lo && sc <= hi;
@@ -1562,7 +1656,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             R = R < 4 ? 4 : R;
         }
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
-        if (H.cfg.limiter == ZK_LIMITER_TVB) A.zsok = H.zsok;
+        if (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
         if (wg)
             hipLaunchKernelGGL((dg_fused_kernel<P, 2, true>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R, nrun);
@@ -1608,9 +1702,19 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB) {
         ProfScope ps(H, "dg_tvb");
-        // the matrix-core kernel when the fused stage certified positivity (any P) or
-        // at P = 3; the per-thread kernel otherwise (faster at P <= 2 without the screen)
-        if (H.no_mfma || (P < 3 && !A.zsok))
+        // after the fused stage (zsok written): a per-thread screen lists the cells that
+        // are limited or not certified, and the matrix-core kernel treats only those;
+        // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
+        if (A.zsok && H.tlist && !H.no_mfma) {
+            hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
+            hipLaunchKernelGGL((dg_tvb_screen_kernel<P>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
+                               H.stream, Qout, H.means, H.src, H.bct, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np,
+                               Mh2, H.tlist, H.scal);
+            const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
+            hipLaunchKernelGGL((dg_tvb_list_kernel<P>), dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
+                               H.stream, Qout, H.means, H.src, H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma,
+                               H.tlist, H.scal);
+        } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
                                H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
         else
@@ -1636,7 +1740,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const uint32_t audit = H.mlp_audit >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(H.mlp_audit * 4294967296.0);
         hipLaunchKernelGGL((dg_learned_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
                            H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.mlp_lo,
-                           H.mlp_hi, audit, H.stage_counter, H.stats, H.scal);
+                           H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.scal);
     }
 }
 
