@@ -16,6 +16,8 @@
 //      positivity scaling (:824-992) of the freshly combined cell.
 // The TVB limiter needs the neighbours' new means, so it runs as its own
 // kernel after the combination, followed by the positivity scaling.
+#include <mutex>
+
 #include "zk_internal.h"
 #include "zk_device.h"
 #include "mlp_device.h"
@@ -1608,6 +1610,8 @@ __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restric
     y[slot] = tv ? 1.0f : 0.0f;
 }
 
+// Constant tables are uploaded once per device (the first DG handle there).
+std::mutex g_tables_mutex;
 bool g_tables_ready[64] = {false};
 
 template <int P>
@@ -1746,16 +1750,24 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
 
 }  // namespace
 
-void dg_upload_tables(int device)
+// Uploads the basis and quadrature tables to the current device (once per
+// device); returns the first HIP error.
+hipError_t dg_upload_tables(int device)
 {
-    if (device >= 0 && device < 64 && g_tables_ready[device]) return;
-    DgTables T[3] = {make_tables(1), make_tables(2), make_tables(3)};
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), T, sizeof(T), 0, hipMemcpyHostToDevice);
+    std::lock_guard<std::mutex> lock(g_tables_mutex);
+    const bool track = device >= 0 && device < 64;
+    if (track && g_tables_ready[device]) return hipSuccess;
+    const DgTables T[3] = {make_tables(1), make_tables(2), make_tables(3)};
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), T, sizeof(T), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
     const MfmaTabs3 M = make_mfma_tables3();
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_mf3), &M, sizeof(M), 0, hipMemcpyHostToDevice);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_mf3), &M, sizeof(M), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
     const MfmaTabsF F[3] = {make_mfma_tables_fused(1), make_mfma_tables_fused(2), make_mfma_tables_fused(3)};
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_mff), F, sizeof(F), 0, hipMemcpyHostToDevice);
-    if (device >= 0 && device < 64) g_tables_ready[device] = true;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_mff), F, sizeof(F), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    if (track) g_tables_ready[device] = true;
+    return hipSuccess;
 }
 
 void dg_stage(Handle &H, const double *Qn, const double *Qin, double *Qout, int stage)

@@ -16,7 +16,7 @@
 #include "zk_internal.h"
 
 namespace zk {
-void dg_upload_tables(int device);
+hipError_t dg_upload_tables(int device);
 zk_status mlp_features_of(Handle &H, zk_mlp *M, const int32_t *cells, int n, uint16_t *X, float *scores);
 }
 
@@ -225,8 +225,13 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     Handle *H = new Handle();
     H->cfg = *cfg;
     H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
+    // measurement overrides: ZK_NO_MFMA=1 (VALU cell kernel), ZK_TUNE=<dg_variant> (a value
+    // zk_set_tuning accepts; anything else is ignored)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
-    if (const char *v = std::getenv("ZK_TUNE")) H->tune = std::atoi(v);
+    if (const char *v = std::getenv("ZK_TUNE")) {
+        const int t = std::atoi(v);
+        if ((t >= 2 && t <= 4) || (t >= 14 && t <= 74) || t == 100 || (t >= 104 && t <= 164)) H->tune = t;
+    }
     e = hipSetDevice(cfg->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -234,7 +239,15 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
         delete H;
         return ZK_ERR_HIP;
     }
-    if (cfg->method == ZK_METHOD_DG) zk::dg_upload_tables(cfg->device);
+    if (cfg->method == ZK_METHOD_DG) {
+        e = zk::dg_upload_tables(cfg->device);
+        if (e != hipSuccess) {
+            g_create_err = std::string("DG table upload: ") + hipGetErrorString(e);
+            (void)hipStreamDestroy(H->stream);
+            delete H;
+            return ZK_ERR_HIP;
+        }
+    }
     *out = reinterpret_cast<zk_handle *>(H);
     return ZK_OK;
 }
