@@ -54,3 +54,27 @@ def test_loss_is_bce_of_sigmoid():
     assert loss == pytest.approx(np.mean(-(y * np.log(s) + (1 - y) * np.log(1 - s))), rel=1e-12)
     np.testing.assert_allclose(p, s)
     assert g[mlp_ref.P_B3] == pytest.approx(np.mean(s - y), rel=1e-12)
+
+
+def test_shipped_network_files():
+    """The shipped DG P=2 network: the .npz (with its training metadata) and
+    the raw float32 vector the zk_dg3 executable reads hold the same
+    parameters, and the oracle's scores with them are probabilities."""
+    from pathlib import Path
+
+    data = Path(__file__).resolve().parents[1] / "zuikakucfd_amd" / "data"
+    d = np.load(data / "troubled_cell_p2.npz")
+    w = d["w"]
+    assert w.shape == (mlp_ref.P_N,) and w.dtype == np.float32
+    raw = np.fromfile(data / "troubled_cell_p2.f32", dtype="<f4")
+    np.testing.assert_array_equal(raw, w)
+    assert int(d["samples"]) > 0 and 0 < int(d["positives"]) < int(d["samples"])
+    assert d["losses"][-1] < d["losses"][0]
+    # the pad columns of W1 (features 180..191) were never trained away from 0
+    W1 = w[mlp_ref.P_W1:mlp_ref.P_B1].reshape(mlp_ref.H, mlp_ref.IN)
+    assert np.all(W1[:, mlp_ref.FEAT:] == 0)
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(32, mlp_ref.IN)) * 0.3
+    X[:, mlp_ref.FEAT:] = 0
+    s = mlp_ref.scores(w.astype(np.float64), X)
+    assert np.isfinite(s).all() and (s >= 0).all() and (s <= 1).all()
