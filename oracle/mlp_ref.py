@@ -70,7 +70,8 @@ def forward_backward(w, X, y, rnd: bool = True):
     a2 = h2pre + b2[:, None]
     h2 = np.maximum(a2, 0.0)
     z = w3 @ h2 + b3
-    p = 1.0 / (1.0 + np.exp(-z))
+    ez = np.exp(-np.abs(z))  # sigmoid without overflow for either sign of z
+    p = np.where(z >= 0, 1.0 / (1.0 + ez), ez / (1.0 + ez))
     loss = float(np.mean(np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z)))))
     dz = (p - y) / n
     g = np.zeros(P_N)
