@@ -1,21 +1,23 @@
 // dg_kernels.hip -- modal DG (P = 1..3) stage kernels for gfx950.
 //
-// A stage is three launches over the update range [3,ny-3)x[3,nx-3):
-//   1. dg_face_kernel<P, X>: one thread per x face, evaluates both traces at
-//      the NQ = P+1 Gauss points of the face, computes the numerical flux once
-//      per point and stores its projection g[c][p] = sum_q w_q L_p(s_q) F*_q[c]
-//      (4*(P+1) doubles per face).  The reference evaluates every face twice,
-//      once from each cell (src_DG/fluid.cpp:731-796); both evaluations see the
-//      same inputs, so one evaluation gives the same flux.
-//   2. the same for y faces (flux called with rho*u and rho*v swapped).
-//   3. dg_cell_kernel<P>: one thread per cell: volume integral at NQ x NQ
-//      Gauss points (src_DG/fluid.cpp:698-718), the four face terms from g
-//      (phi_m(+-1, s) = (+-1)^px L_py(s) makes each face term a signed copy of
-//      g), the invMass(m) * 0.5/h scale (:800-804), the Shu-Osher combination
-//      (:1022-1053) and, for the reference limiter, the Persson filter and the
-//      positivity scaling (:824-992) of the freshly combined cell.
-// The TVB limiter needs the neighbours' new means, so it runs as its own
-// kernel after the combination, followed by the positivity scaling.
+// A stage combines the volume integral (src_DG/fluid.cpp:698-718), the four
+// face terms (:731-796), the invMass(m) * 0.5/h scale (:800-804) and the
+// Shu-Osher combination (:1022-1053), then the limiter (:824-992, or the TVB
+// limiter of this project).  Two kernel structures compute it:
+//   * fused (default at P >= 2 without the Persson limiter): dg_fused_kernel
+//     marches rows of 16-column runs on FP64 matrix cores; every face flux is
+//     computed once, the state is read about once per stage, and the kernel
+//     certifies cells whose positivity scaling is a no-op (zsok).  In its
+//     workgroup form the matrix-core tables sit in LDS and each row arrives
+//     by global-to-LDS loads issued while the previous row is computed;
+//   * split: dg_face_kernel<P, X> computes each face's projected flux g[c][p]
+//     once (the reference evaluates every face twice, from each cell; both see
+//     the same inputs), then dg_cell_kernel<P> (VALU, also the Persson path)
+//     or dg_cell_mfma3_kernel (P = 3) forms the update.
+// The TVB limiter needs the neighbours' new means, so it runs after the stage:
+// a per-thread screen lists the cells that are limited or not certified, and a
+// matrix-core kernel limits and applies the positivity scaling to those.
+// The learned indicator (P = 2) scores only the TVB candidate cells.
 #include <mutex>
 
 #include "zk_internal.h"
@@ -852,19 +854,34 @@ MfmaTabsF make_mfma_tables_fused(int P)
     return M;
 }
 
+// Workgroup barrier for the LDS exchanges of the fused kernel.  It waits for
+// this wave's LDS accesses only: __syncthreads would also wait for the
+// global-to-LDS row loads in flight (they count as vector memory operations).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // WG = false: every wave is independent and owns 16 columns, 14 of them
 // updated.  WG = true: the four waves of a workgroup march the same rows side
 // by side over 64 columns, 62 of them updated; the x faces between waves take
 // the neighbour wave's boundary trace and face projection from LDS (two
 // barriers per row), so halo columns exist only at workgroup edges.
-template <int P, int MINW, bool WG>
+// MODE (with WG): 0 reads the matrix-core tables from constant memory every
+// row; 1 copies them to LDS once; 2 also brings the rows to LDS by direct
+// global-to-LDS loads issued one row ahead (wave w loads the planes of
+// variable w), so the next row is in flight while this one is computed,
+// without holding registers for it.
+template <int P, int MINW, bool WG, int MODE>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
-    const MfmaTabsF &M = c_mff[P - 1];
+    constexpr bool DMA = MODE == 2;
+    static_assert(WG || MODE == 0, "LDS tables need the workgroup form");
+    __shared__ double tabs_lds[MODE >= 1 ? sizeof(MfmaTabsF) / 8 : 1];
+    const MfmaTabsF &M = (MODE >= 1) ? *reinterpret_cast<const MfmaTabsF *>(tabs_lds) : c_mff[P - 1];
     const DgTables &T = c_tab[P - 1];
     __shared__ double lds_w[WG ? 4 : 1][4][4];   // [wave][g][c]: W trace of the wave's column 0
     __shared__ double lds_ge[WG ? 4 : 1][4][4];  // [wave][g][c]: projected flux of its column-15 face
+    __shared__ double qbuf[DMA ? 4 * NM : 1][DMA ? 64 : 1];  // one row: plane (c, mode) x 64 columns
+    __shared__ int32_t srcbuf[DMA ? 2 : 1][DMA ? 64 : 1];     // ghost-map entries of rows r (r & 1)
     if (A.S->done) return;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
@@ -886,6 +903,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         seg = w / nrun;
         j = 2 + run * 14 + col;
         store_col = col >= 1 && col <= 14 && j <= A.nx - 4;
+    }
+    if (MODE >= 1) {
+        const double *src = reinterpret_cast<const double *>(&c_mff[P - 1]);
+        for (int i = threadIdx.x; i < (int)(sizeof(MfmaTabsF) / 8); i += 256) tabs_lds[i] = src[i];
+        __syncthreads();
     }
     const int jc = j < A.nx ? j : A.nx - 1;           // clamped for loads past the right edge
     const int r0 = 3 + seg * R;
@@ -922,6 +944,47 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             for (int v = 0; v < 4; ++v) Q[2][v] = -Q[2][v];
     };
 
+    // Row staging for DMA.  Lane l of wave w moves dword (l & 1) of columns
+    // (l >> 1) and 32 + (l >> 1) of every plane of variable w; the source
+    // cells come from the ghost map (slip signs are applied on reading).
+    // Wave 0 also moves the ghost-map row two ahead into srcbuf.
+    const int jrun0 = 2 + run * 62;
+    const int wvu = __builtin_amdgcn_readfirstlane(wv);
+    auto col_src = [&](int row, int x) {
+        const int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
+        return A.src[(size_t)row * A.nx + jj];
+    };
+    auto dma_row = [&](int32_t e0, int32_t e1) {
+        const uint32_t b = (uint32_t)(lane & 1) * 4u;
+        const uint32_t o0 = (uint32_t)(e0 >> kSrcShift) * 8u + b, o1 = (uint32_t)(e1 >> kSrcShift) * 8u + b;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const int pl = wvu * NM + m;
+            const char *plane = reinterpret_cast<const char *>(A.Qin + (size_t)pl * np);
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o0),
+                                             (void __attribute__((address_space(3))) *)&qbuf[pl][0], 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o1),
+                                             (void __attribute__((address_space(3))) *)&qbuf[pl][32], 4, 0, 0);
+        }
+    };
+    auto dma_src = [&](int row) {
+        const int32_t *rowp = A.src + (size_t)row * A.nx;  // wave-uniform
+        // the lane index is re-derived here rather than kept live across the row
+        int l;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+        const uint32_t js = (uint32_t)(jrun0 + l < A.nx ? jrun0 + l : A.nx - 1);
+        __builtin_amdgcn_global_load_lds(
+            (const void __attribute__((address_space(1))) *)(rowp + js),
+            (void __attribute__((address_space(3))) *)&srcbuf[row & 1][0], 4, 0, 0);
+    };
+    if (DMA) {
+        dma_row(col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)));
+        if (wvu == 0) {
+            dma_src(r0);
+            if (r0 + 1 <= r1) dma_src(r0 + 1);
+        }
+    }
+
     // S trace of the halo row above the segment
     double Sup[4];
     {
@@ -955,7 +1018,31 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
         double Q[4][4];
-        load(r, Q);
+        if (DMA) {
+            // row r is complete in qbuf once every wave's loads have landed
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            const int x = 16 * wv + col;
+            const int32_t eo = srcbuf[r & 1][x];
+            const int gq = slot ? g : 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Q[c][v] = (v <= P && slot) ? qbuf[c * NM + v * NQ + gq][x] : 0.0;
+            if (eo & 1)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Q[1][v] = -Q[1][v];
+            if (eo & 2)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Q[2][v] = -Q[2][v];
+            __syncthreads();  // qbuf is free for the next row
+            if (inner) {
+                dma_row(srcbuf[(r + 1) & 1][lane >> 1], srcbuf[(r + 1) & 1][32 + (lane >> 1)]);
+                if (wvu == 0 && r + 2 <= r1) dma_src(r + 2);
+            }
+        } else {
+            load(r, Q);
+        }
         // face traces of this row: element side (W, E, N, S) at quadrature point g
         d4 Tr[4];
 #pragma unroll
@@ -1000,11 +1087,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             // Sufficient condition for the positivity scaling to leave the cell
             // alone: lower bounds of rho and p over its 16 points (means minus
             // the mode sums) clear the 1e-5 floors with a margin.
+            // sums over the four lane groups land in group 0, which holds the means
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                sab[c] += __shfl_xor(sab[c], 16);
-                sab[c] += __shfl_xor(sab[c], 32);
-                mean0[c] = __shfl(mean0[c], col);
+                sab[c] += __shfl_down(sab[c], 32);
+                sab[c] += __shfl_down(sab[c], 16);
             }
             if (g == 0 && store_col) {
                 const double rlo = mean0[0] - sab[0];
@@ -1032,7 +1119,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         d4 acc[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double sN = T.sgn[g < NQ ? g : 0];
+            const double sN = (g & 1) ? -1.0 : 1.0;  // L_g(-1)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[c][v] = sN * gN[c][v];
         }
@@ -1048,7 +1135,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 if (col == 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_w[wv][g][c] = Tr[c][0];
-                __syncthreads();
+                lds_barrier();
                 if (col == 15 && wv < 3)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) Rx[c] = lds_w[wv + 1][g][c];
@@ -1066,7 +1153,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 if (col == 15)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_ge[wv][g][c] = gE[c];
-                __syncthreads();
+                lds_barrier();
                 if (col == 0 && wv > 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wv - 1][g][c];
@@ -1074,7 +1161,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[c][v] += T.sgn[v < NQ ? v : 0] * gW[c] - gE[c];
+                for (int v = 0; v < 4; ++v) acc[c][v] += ((v & 1) ? -gW[c] : gW[c]) - gE[c];  // L_v(-1) gW - gE
         }
         {
             d4 U[4];
@@ -1662,10 +1749,17 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
         if (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
-        if (wg)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R, nrun);
+        if (wg && H.dma == 2)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 2>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
+                               nrun);
+        else if (wg && H.dma == 1)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 1>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
+                               nrun);
+        else if (wg)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 0>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
+                               nrun);
         else
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false, 0>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
                                H.stream, A, R, nrun);
     } else if (ncell > 0) {
         {
