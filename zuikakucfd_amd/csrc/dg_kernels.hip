@@ -181,7 +181,11 @@ struct DgArgs {
     const StepScalars *S;
     double *gx, *gy;
     double *means;    // stage means (4 planes) for the TVB limiter, or null
-    uint8_t *zsok;    // fused kernel: 1 where the positivity scaling is a no-op (TVB limiters), or null
+    // fused kernel under the TVB limiters, or null: bit 0 set where the
+    // positivity scaling is certified to be a no-op, bit 1 where the cell is
+    // a TVB candidate (a linear mode above Mh2, or NaN)
+    uint8_t *zsok;
+    double Mh2;
     int ny, nx, nxu, nyu;
     size_t np, nfx, nfy;
     double da, gamma;
@@ -868,20 +872,22 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // row; 1 copies them to LDS once; 2 also brings the rows to LDS by direct
 // global-to-LDS loads issued one row ahead (wave w loads the planes of
 // variable w), so the next row is in flight while this one is computed,
-// without holding registers for it.
+// without holding registers for it; 3 is 2 with two row buffers, which
+// saves the barrier that frees the single buffer.
 template <int P, int MINW, bool WG, int MODE>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
-    constexpr bool DMA = MODE == 2;
+    constexpr bool DMA = MODE >= 2, DB = MODE == 3;
+    constexpr int NB = DB ? 2 : 1, NS = DB ? 4 : 2;  // row buffers, ghost-map row slots
     static_assert(WG || MODE == 0, "LDS tables need the workgroup form");
     __shared__ double tabs_lds[MODE >= 1 ? sizeof(MfmaTabsF) / 8 : 1];
     const MfmaTabsF &M = (MODE >= 1) ? *reinterpret_cast<const MfmaTabsF *>(tabs_lds) : c_mff[P - 1];
     const DgTables &T = c_tab[P - 1];
     __shared__ double lds_w[WG ? 4 : 1][4][4];   // [wave][g][c]: W trace of the wave's column 0
     __shared__ double lds_ge[WG ? 4 : 1][4][4];  // [wave][g][c]: projected flux of its column-15 face
-    __shared__ double qbuf[DMA ? 4 * NM : 1][DMA ? 64 : 1];  // one row: plane (c, mode) x 64 columns
-    __shared__ int32_t srcbuf[DMA ? 2 : 1][DMA ? 64 : 1];     // ghost-map entries of rows r (r & 1)
+    __shared__ double qbuf[DMA ? NB : 1][DMA ? 4 * NM : 1][DMA ? 64 : 1];  // rows: plane (c, mode) x 64 columns
+    __shared__ int32_t srcbuf[DMA ? NS : 1][DMA ? 64 : 1];               // ghost-map entries of row r: slot r % NS
     if (A.S->done) return;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
@@ -954,7 +960,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
         return A.src[(size_t)row * A.nx + jj];
     };
-    auto dma_row = [&](int32_t e0, int32_t e1) {
+    auto dma_row = [&](int row, int32_t e0, int32_t e1) {
+        auto qb = qbuf[DB ? (row & 1) : 0];
         const uint32_t b = (uint32_t)(lane & 1) * 4u;
         const uint32_t o0 = (uint32_t)(e0 >> kSrcShift) * 8u + b, o1 = (uint32_t)(e1 >> kSrcShift) * 8u + b;
 #pragma unroll
@@ -962,9 +969,9 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             const int pl = wvu * NM + m;
             const char *plane = reinterpret_cast<const char *>(A.Qin + (size_t)pl * np);
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o0),
-                                             (void __attribute__((address_space(3))) *)&qbuf[pl][0], 4, 0, 0);
+                                             (void __attribute__((address_space(3))) *)&qb[pl][0], 4, 0, 0);
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o1),
-                                             (void __attribute__((address_space(3))) *)&qbuf[pl][32], 4, 0, 0);
+                                             (void __attribute__((address_space(3))) *)&qb[pl][32], 4, 0, 0);
         }
     };
     auto dma_src = [&](int row) {
@@ -975,10 +982,10 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const uint32_t js = (uint32_t)(jrun0 + l < A.nx ? jrun0 + l : A.nx - 1);
         __builtin_amdgcn_global_load_lds(
             (const void __attribute__((address_space(1))) *)(rowp + js),
-            (void __attribute__((address_space(3))) *)&srcbuf[row & 1][0], 4, 0, 0);
+            (void __attribute__((address_space(3))) *)&srcbuf[row % NS][0], 4, 0, 0);
     };
     if (DMA) {
-        dma_row(col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)));
+        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)));
         if (wvu == 0) {
             dma_src(r0);
             if (r0 + 1 <= r1) dma_src(r0 + 1);
@@ -1022,22 +1029,29 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             // row r is complete in qbuf once every wave's loads have landed
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            const int x = 16 * wv + col;
-            const int32_t eo = srcbuf[r & 1][x];
-            const int gq = slot ? g : 0;
+            // LDS addresses come from the opaque lane copy, so they are
+            // recomputed per row instead of being held (or spilled) across it
+            const int x = 16 * wvu + (ln & 15);
+            const int32_t eo = srcbuf[r % NS][x];
+            auto qb = qbuf[DB ? (r & 1) : 0];
+            const int gq = (ln >> 4) <= P ? (ln >> 4) : 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Q[c][v] = (v <= P && slot) ? qbuf[c * NM + v * NQ + gq][x] : 0.0;
+                for (int v = 0; v < 4; ++v) Q[c][v] = (v <= P && slot) ? qb[c * NM + v * NQ + gq][x] : 0.0;
             if (eo & 1)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Q[1][v] = -Q[1][v];
             if (eo & 2)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Q[2][v] = -Q[2][v];
-            __syncthreads();  // qbuf is free for the next row
+            // the single buffer is free for the next row once every wave has
+            // read it; with two buffers the next row goes to the other one,
+            // last read in the previous iteration (before this row's barrier)
+            if (!DB) __syncthreads();
             if (inner) {
-                dma_row(srcbuf[(r + 1) & 1][lane >> 1], srcbuf[(r + 1) & 1][32 + (lane >> 1)]);
+                const int32_t *sn = srcbuf[(r + 1) % NS];
+                dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)]);
                 if (wvu == 0 && r + 2 <= r1) dma_src(r + 2);
             }
         } else {
@@ -1067,6 +1081,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         }
         // sab[c]: sum over the non-mean modes of |mode| * max |basis| at the positivity points
         double sab[4] = {0.0, 0.0, 0.0, 0.0}, mean0[4] = {0.0, 0.0, 0.0, 0.0};
+        bool lin_big = false;  // this lane's linear mode (x in group 0, y in group 1) is above M h^2, or NaN
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
@@ -1081,6 +1096,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     } else {
                         sab[c] += fabs(out) * M.zc[4 * v + g];
                     }
+                    if ((v == 1 && g == 0) || (v == 0 && g == 1)) lin_big = lin_big || !(fabs(out) <= A.Mh2);
                 }
         }
         if (A.zsok && r > r0) {
@@ -1093,6 +1109,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 sab[c] += __shfl_down(sab[c], 32);
                 sab[c] += __shfl_down(sab[c], 16);
             }
+            int cand = lin_big ? 1 : 0;
+            cand |= __shfl_down(cand, 16);  // group 1's y-linear test
             if (g == 0 && store_col) {
                 const double rlo = mean0[0] - sab[0];
                 const double elo = mean0[3] - sab[3];
@@ -1103,7 +1121,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     const double plo = (gam - 1.0) * (elo - 0.5 * (mx * mx + my * my) / rlo);
                     ok = plo > 2e-5 + tol;
                 }
-                A.zsok[kp] = (uint8_t)ok;
+                A.zsok[kp] = (uint8_t)((ok ? 1 : 0) | (cand ? 2 : 0));
             }
         }
         if (!inner) break;
@@ -1134,11 +1152,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             if (WG) {
                 if (col == 0)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) lds_w[wv][g][c] = Tr[c][0];
+                    for (int c = 0; c < 4; ++c) lds_w[wvu][ln >> 4][c] = Tr[c][0];
                 lds_barrier();
                 if (col == 15 && wv < 3)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) Rx[c] = lds_w[wv + 1][g][c];
+                    for (int c = 0; c < 4; ++c) Rx[c] = lds_w[wvu + 1][ln >> 4][c];
             }
             if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
             else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
@@ -1152,11 +1170,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             if (WG) {
                 if (col == 15)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) lds_ge[wv][g][c] = gE[c];
+                    for (int c = 0; c < 4; ++c) lds_ge[wvu][ln >> 4][c] = gE[c];
                 lds_barrier();
                 if (col == 0 && wv > 0)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wv - 1][g][c];
+                    for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wvu - 1][ln >> 4][c];
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -1286,7 +1304,7 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     }
     chg = __shfl(chg, col);
     {
-        const bool sure = zsok && zsok[k];  // positivity scaling certified to be a no-op
+        const bool sure = zsok && (zsok[k] & 1);  // positivity scaling certified to be a no-op
         const bool need = live && (chg != 0 || !sure);
         if (!__any(need)) {
             if (live && g == 0) flags[k] = 0;
@@ -1412,48 +1430,40 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live);
 }
 
-// TVB screen, one thread per cell: the TVB test from the mean and linear
-// modes; a cell that is unlimited and certified by zsok gets flag 0 and is
-// done, the others are listed (list[0] = count) for dg_tvb_list_kernel.
-template <int P>
-__global__ __launch_bounds__(256) void dg_tvb_screen_kernel(const double *__restrict__ Q, const double *__restrict__ Mn,
-                                                            const int32_t *__restrict__ src,
-                                                            const int32_t *__restrict__ bct,
-                                                            uint8_t *__restrict__ flags,
-                                                            const uint8_t *__restrict__ zsok, int nx, int nxu,
-                                                            size_t ncell, size_t np, double Mh2,
-                                                            int32_t *__restrict__ list,
-                                                            const StepScalars *__restrict__ S)
+// Lists the cells the limiter pass must treat, from the byte the fused stage
+// kernel leaves per cell (bit 0: positivity scaling certified to be a no-op,
+// bit 1: TVB candidate) and the updated-cell mask.  With tvb: the updated
+// candidates and the uncertified cells, the others get flag 0 and are done;
+// otherwise the updated candidates only, added to *count.  One thread per
+// cell, one atomic per wavefront; list[0] must be zero on entry.
+__global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict__ zsok,
+                                                      const uint8_t *__restrict__ upd, uint8_t *__restrict__ flags,
+                                                      int nx, int nxu, size_t ncell, int tvb,
+                                                      int32_t *__restrict__ list,
+                                                      unsigned long long *__restrict__ count,
+                                                      const StepScalars *__restrict__ S)
 {
-    constexpr int NQ = P + 1, NM = NQ * NQ;
     if (S->done) return;  // uniform over the grid
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool need = false;
     size_t k = 0;
     if (t < ncell) {
         k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
-        bool chg_any = false;
-        if (bct[k] == 0) {
-            double C[4][NM];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                C[c][0] = Q[(size_t)(c * NM) * np + k];
-                C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
-                C[c][NQ] = Q[(size_t)(c * NM + NQ) * np + k];
-            }
-            double axl[4], ayl[4];
-            bool chg[4];
-            chg_any = tvb_test<P>(C, Mn, src, np, k, nx, Mh2, axl, ayl, chg);
-        }
-        need = chg_any || !zsok[k];
-        if (!need) flags[k] = 0;
+        const unsigned z = zsok[k];
+        const bool cand = (z & 2) && upd[k];
+        need = cand || (tvb && !(z & 1));
+        if (tvb && !need) flags[k] = 0;
     }
     const unsigned long long mask = __ballot(need);
     if (!mask) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)mask) - 1;
+    const int n = __popcll(mask);
     int base = 0;
-    if (lane == leader) base = atomicAdd(list, __popcll(mask));
+    if (lane == leader) {
+        base = atomicAdd(list, n);
+        if (count) atomicAdd(count, (unsigned long long)n);
+    }
     base = __shfl(base, leader);
     if (need) list[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
 }
@@ -1600,7 +1610,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
         C[c][NQ] = Q[(size_t)(c * NM + NQ) * np + k];
     }
     const bool cand = bct[k] == 0 && tvb_candidate<P>(C, Mh2);
-    if (!cand && zsok && zsok[k]) {
+    if (!cand && zsok && (zsok[k] & 1)) {
         flags[k] = 0;
         return;
     }
@@ -1710,6 +1720,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.gx = H.gx; A.gy = H.gy;
     A.means = (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) ? H.means : nullptr;
     A.zsok = nullptr;  // set below when the fused kernel runs under the TVB limiter
+    A.Mh2 = H.cfg.tvb_M * H.g.da * H.g.da;
     A.ny = H.g.ny; A.nx = H.g.nx; A.nxu = H.g.nx - 6; A.nyu = H.g.ny - 6;
     A.np = H.g.np;
     A.nfx = (size_t)(A.nxu + 1) * A.nyu;
@@ -1749,7 +1760,10 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
         if (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
-        if (wg && H.dma == 2)
+        if (wg && H.dma == 3)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
+                               nrun);
+        else if (wg && H.dma == 2)
             hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 2>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
                                nrun);
         else if (wg && H.dma == 1)
@@ -1788,7 +1802,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         hipLaunchKernelGGL(dg_ring_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, H.stream, Qn, Qin, Qout,
                            H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means);
     }
-    const double Mh2 = H.cfg.tvb_M * H.g.da * H.g.da;
+    const double Mh2 = A.Mh2;
     H.stage_counter += 1;
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB && H.harvest && P == 2) {
         ProfScope ps(H, "dg_harvest");
@@ -1805,9 +1819,9 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
         if (A.zsok && H.tlist && !H.no_mfma) {
             hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
-            hipLaunchKernelGGL((dg_tvb_screen_kernel<P>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
-                               H.stream, Qout, H.means, H.src, H.bct, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np,
-                               Mh2, H.tlist, H.scal);
+            hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream, A.zsok,
+                               H.upd, H.flags, H.g.nx, A.nxu, ncell, 1, H.tlist, (unsigned long long *)nullptr,
+                               H.scal);
             const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
             hipLaunchKernelGGL((dg_tvb_list_kernel<P>), dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
                                H.stream, Qout, H.means, H.src, H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma,
@@ -1824,8 +1838,13 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         {
             ProfScope ps(H, "dg_candidates");
             hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.cand);
-            hipLaunchKernelGGL((dg_cand_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout, H.bct,
-                               H.g.nx, A.nxu, ncell, H.g.np, Mh2, H.cand, H.stats, H.scal);
+            if (A.zsok)  // the fused kernel marked the candidates
+                hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream,
+                                   A.zsok, H.upd, (uint8_t *)nullptr, H.g.nx, A.nxu, ncell, 0, H.cand, H.stats + 6,
+                                   H.scal);
+            else
+                hipLaunchKernelGGL((dg_cand_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout,
+                                   H.bct, H.g.nx, A.nxu, ncell, H.g.np, Mh2, H.cand, H.stats, H.scal);
         }
         {
             ProfScope ps(H, "dg_score");

@@ -229,7 +229,7 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     // zk_set_tuning accepts; anything else is ignored)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
     if (const char *v = std::getenv("ZK_DMA"))
-        if (v[0] >= '0' && v[0] <= '2' && v[1] == 0) H->dma = v[0] - '0';
+        if (v[0] >= '0' && v[0] <= '3' && v[1] == 0) H->dma = v[0] - '0';
     if (const char *v = std::getenv("ZK_TUNE")) {
         const int t = std::atoi(v);
         if ((t >= 2 && t <= 4) || (t >= 14 && t <= 74) || t == 100 || (t >= 104 && t <= 164)) H->tune = t;

@@ -45,8 +45,9 @@ struct Handle {
     bool loaded = false;
     bool no_mfma = false;     // force the VALU cell kernel (ZK_NO_MFMA=1), for comparison
     int tune = 0;             // kernel variant selector (ZK_TUNE), used by the autotuner
-    int dma = 2;              // workgroup fused DG kernel: 0 tables from constant memory, 1 tables in LDS,
-                              // 2 tables in LDS and rows by global-to-LDS loads (ZK_DMA overrides)
+    int dma = 3;              // workgroup fused DG kernel: 0 tables from constant memory, 1 tables in LDS,
+                              // 2 tables in LDS and rows by global-to-LDS loads, 3 the same with two
+                              // row buffers (ZK_DMA overrides)
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
     bool use_graph = true;    // WENO zk_step: replay one captured step as a HIP graph
     hipGraph_t step_graph = nullptr;
