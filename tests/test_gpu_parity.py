@@ -80,6 +80,32 @@ def test_weno_graph_replays_match_eager():
     e.close()
 
 
+@pytest.mark.parametrize("P", [2, 3])
+def test_dg_graph_replays_match_eager(P):
+    """zk_step replays a captured DG TVB step too: replay batches, with eager
+    work between them and a variant change that forces a new capture, equal
+    the same steps launched eagerly, bit for bit (flags included)."""
+    say(f"dg P={P} graph replay vs eager, random mask 53")
+    m = meshio.random_riemann(5, 53)
+    g = zk.Solver(method=zk.METHOD_DG, P=P, limiter=zk.LIMITER_TVB)
+    e = zk.Solver(method=zk.METHOD_DG, P=P, limiter=zk.LIMITER_TVB)
+    for s in (g, e):
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    e.set_tuning(0, 0, use_graph=False)
+    dt = 0.2 * m.da / max(e.max_speed(), 1e-7) / (2 * P + 1)
+    for rep in range(4):
+        if rep == 2:
+            g.set_tuning(0, 104, use_graph=True)  # other segment length: new capture
+            e.set_tuning(0, 104, use_graph=False)
+        g.step(dt, 3)
+        e.step(dt, 3)
+        assert g.max_speed() == e.max_speed()
+        np.testing.assert_array_equal(g.get_state(), e.get_state())
+        np.testing.assert_array_equal(g.get_flags(), e.get_flags())
+    g.close()
+    e.close()
+
+
 def test_weno_max_speed():
     say("weno max speed")
     m = meshio.riemann_2d(64)

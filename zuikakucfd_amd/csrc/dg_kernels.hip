@@ -960,14 +960,14 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
         return A.src[(size_t)row * A.nx + jj];
     };
-    auto dma_row = [&](int row, int32_t e0, int32_t e1) {
+    auto dma_row = [&](int row, int32_t e0, int32_t e1, size_t npl) {
         auto qb = qbuf[DB ? (row & 1) : 0];
         const uint32_t b = (uint32_t)(lane & 1) * 4u;
         const uint32_t o0 = (uint32_t)(e0 >> kSrcShift) * 8u + b, o1 = (uint32_t)(e1 >> kSrcShift) * 8u + b;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             const int pl = wvu * NM + m;
-            const char *plane = reinterpret_cast<const char *>(A.Qin + (size_t)pl * np);
+            const char *plane = reinterpret_cast<const char *>(A.Qin + (size_t)pl * npl);
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o0),
                                              (void __attribute__((address_space(3))) *)&qb[pl][0], 4, 0, 0);
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o1),
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             (void __attribute__((address_space(3))) *)&srcbuf[row % NS][0], 4, 0, 0);
     };
     if (DMA) {
-        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)));
+        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), np);
         if (wvu == 0) {
             dma_src(r0);
             if (r0 + 1 <= r1) dma_src(r0 + 1);
@@ -1022,6 +1022,10 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // them in registers across the loop
         int ln = lane;
         asm volatile("" : "+v"(ln));
+        // likewise the plane stride: the plane addresses are formed per row
+        // (scalar arithmetic) instead of being held across the loop
+        size_t npr = np;
+        asm volatile("" : "+s"(npr));
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
         double Q[4][4];
@@ -1051,7 +1055,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             if (!DB) __syncthreads();
             if (inner) {
                 const int32_t *sn = srcbuf[(r + 1) % NS];
-                dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)]);
+                dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)], npr);
                 if (wvu == 0 && r + 2 <= r1) dma_src(r + 2);
             }
         } else {
@@ -1089,7 +1093,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v <= P; ++v) {
                     const double out = base[c][v] - fs[v] * gN[c][v];
-                    (A.Qout + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)kp] = out;
+                    (A.Qout + (size_t)(c * NM + v * NQ) * npr)[gofs + (uint32_t)kp] = out;
                     if (v == 0 && g == 0) {
                         mean0[c] = out;
                         if (A.means) A.means[(size_t)c * np + kp] = out;
@@ -1131,7 +1135,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v <= P; ++v) qn[c][v] = (A.Qn + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+                for (int v = 0; v <= P; ++v) qn[c][v] = (A.Qn + (size_t)(c * NM + v * NQ) * npr)[gofs + (uint32_t)k];
         // rhs of row r, all terms but its S face: N face, x faces, then the volume
         // projection accumulated on top by the MFMA
         d4 acc[4];
@@ -1432,29 +1436,15 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
 
 // Lists the cells the limiter pass must treat, from the byte the fused stage
 // kernel leaves per cell (bit 0: positivity scaling certified to be a no-op,
-// bit 1: TVB candidate) and the updated-cell mask.  With tvb: the updated
-// candidates and the uncertified cells, the others get flag 0 and are done;
-// otherwise the updated candidates only, added to *count.  One thread per
-// cell, one atomic per wavefront; list[0] must be zero on entry.
-__global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict__ zsok,
-                                                      const uint8_t *__restrict__ upd, uint8_t *__restrict__ flags,
-                                                      int nx, int nxu, size_t ncell, int tvb,
-                                                      int32_t *__restrict__ list,
-                                                      unsigned long long *__restrict__ count,
-                                                      const StepScalars *__restrict__ S)
+// bit 1: TVB candidate) and the updated-cell mask.  list gets the updated
+// candidates and the uncertified cells; all other cells get flag 0 and are
+// done.  cand (if not null) gets the updated candidates alone, counted in
+// *count.  One thread per cell, one atomic per list and wavefront; the
+// counts (entry 0) must be zero on entry.
+__device__ __forceinline__ void wave_append(bool in, int32_t *__restrict__ list, unsigned long long *count,
+                                            int32_t k)
 {
-    if (S->done) return;  // uniform over the grid
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool need = false;
-    size_t k = 0;
-    if (t < ncell) {
-        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
-        const unsigned z = zsok[k];
-        const bool cand = (z & 2) && upd[k];
-        need = cand || (tvb && !(z & 1));
-        if (tvb && !need) flags[k] = 0;
-    }
-    const unsigned long long mask = __ballot(need);
+    const unsigned long long mask = __ballot(in);
     if (!mask) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)mask) - 1;
@@ -1465,7 +1455,29 @@ __global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict_
         if (count) atomicAdd(count, (unsigned long long)n);
     }
     base = __shfl(base, leader);
-    if (need) list[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
+    if (in) list[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+}
+
+__global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict__ zsok,
+                                                      const uint8_t *__restrict__ upd, uint8_t *__restrict__ flags,
+                                                      int nx, int nxu, size_t ncell, int32_t *__restrict__ list,
+                                                      int32_t *__restrict__ cand,
+                                                      unsigned long long *__restrict__ count,
+                                                      const StepScalars *__restrict__ S)
+{
+    if (S->done) return;  // uniform over the grid
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false, is_cand = false;
+    size_t k = 0;
+    if (t < ncell) {
+        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+        const unsigned z = zsok[k];
+        is_cand = (z & 2) && upd[k];
+        need = is_cand || !(z & 1);
+        if (!need) flags[k] = 0;
+    }
+    wave_append(need, list, nullptr, (int32_t)k);
+    if (cand) wave_append(is_cand, cand, count, (int32_t)k);
 }
 
 // Full TVB + positivity work on the listed cells, 16 per wave, persistent grid.
@@ -1578,28 +1590,17 @@ __global__ __launch_bounds__(256) void dg_score_list_kernel(const double *__rest
     }
 }
 
-// Learned limiter.  Non-candidate cells pass unchanged, as they pass the TVB
-// test.  For candidates, outside the uncertainty band [lo, hi] the score
-// decides (troubled = score > hi, then every variable is limited); inside the
-// band and on a random audit fraction the plain TVB test decides.  Audited
-// cells also record agreement of the network's 0.5 decision with the TVB
-// test.  The positivity scaling follows, as in the TVB limiter.
+// One cell of the learned limiter (see dg_learned_kernel).
 template <int P>
-__global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q, const double *__restrict__ M,
-                                                         const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
-                                                         uint8_t *__restrict__ flags, const float *__restrict__ score,
-                                                         int ny, int nx, size_t np, double Mh2, double gamma, float lo,
-                                                         float hi, uint32_t audit_thresh, uint32_t salt,
-                                                         unsigned long long *__restrict__ stats,
-                                                         const uint8_t *__restrict__ zsok,
-                                                         const StepScalars *__restrict__ S)
+__device__ __forceinline__ void learned_cell(double *__restrict__ Q, const double *__restrict__ M,
+                                             const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                             uint8_t *__restrict__ flags, const float *__restrict__ score, int nx,
+                                             size_t np, double Mh2, double gamma, float lo, float hi,
+                                             uint32_t audit_thresh, uint32_t salt,
+                                             unsigned long long *__restrict__ stats,
+                                             const uint8_t *__restrict__ zsok, size_t k)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
-    const int nxu = nx - 6, nyu = ny - 6;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nxu * nyu || S->done) return;
-    const int i = 3 + (int)(t / nxu), j = 3 + (int)(t % nxu);
-    const size_t k = (size_t)i * nx + j;
     double C[4][NM];
     // the mean and linear modes first: a non-candidate cell whose positivity
     // scaling is certified a no-op (zsok) is left as it is
@@ -1666,6 +1667,35 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
         for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+    }
+}
+
+// Learned limiter.  Non-candidate cells pass unchanged, as they pass the TVB
+// test.  For candidates, outside the uncertainty band [lo, hi] the score
+// decides (troubled = score > hi, then every variable is limited); inside the
+// band and on a random audit fraction the plain TVB test decides.  Audited
+// cells also record agreement of the network's 0.5 decision with the TVB
+// test.  The positivity scaling follows, as in the TVB limiter.
+// LIST: the threads walk list[1..list[0]] (grid-stride) instead of the grid;
+// the cells not listed have been finished by the list pass.
+template <int P, bool LIST>
+__global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q, const double *__restrict__ M,
+                                                         const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                                         uint8_t *__restrict__ flags, const float *__restrict__ score,
+                                                         int ny, int nx, size_t np, double Mh2, double gamma, float lo,
+                                                         float hi, uint32_t audit_thresh, uint32_t salt,
+                                                         unsigned long long *__restrict__ stats,
+                                                         const uint8_t *__restrict__ zsok,
+                                                         const int32_t *__restrict__ list,
+                                                         const StepScalars *__restrict__ S)
+{
+    if (S->done) return;
+    const int nxu = nx - 6, nyu = ny - 6;
+    const size_t n = LIST ? (size_t)list[0] : (size_t)nxu * nyu;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = LIST ? (size_t)list[1 + t] : (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+        learned_cell<P>(Q, M, src, bct, flags, score, nx, np, Mh2, gamma, lo, hi, audit_thresh, salt, stats,
+                        LIST ? nullptr : zsok, k);
     }
 }
 
@@ -1820,8 +1850,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         if (A.zsok && H.tlist && !H.no_mfma) {
             hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
             hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream, A.zsok,
-                               H.upd, H.flags, H.g.nx, A.nxu, ncell, 1, H.tlist, (unsigned long long *)nullptr,
-                               H.scal);
+                               H.upd, H.flags, H.g.nx, A.nxu, ncell, H.tlist, (int32_t *)nullptr,
+                               (unsigned long long *)nullptr, H.scal);
             const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
             hipLaunchKernelGGL((dg_tvb_list_kernel<P>), dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
                                H.stream, Qout, H.means, H.src, H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma,
@@ -1838,13 +1868,14 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         {
             ProfScope ps(H, "dg_candidates");
             hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.cand);
-            if (A.zsok)  // the fused kernel marked the candidates
+            if (A.zsok) {  // the fused kernel marked the candidates: list them and the uncertified cells
+                hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
                 hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream,
-                                   A.zsok, H.upd, (uint8_t *)nullptr, H.g.nx, A.nxu, ncell, 0, H.cand, H.stats + 6,
-                                   H.scal);
-            else
+                                   A.zsok, H.upd, H.flags, H.g.nx, A.nxu, ncell, H.tlist, H.cand, H.stats + 6, H.scal);
+            } else {
                 hipLaunchKernelGGL((dg_cand_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout,
                                    H.bct, H.g.nx, A.nxu, ncell, H.g.np, Mh2, H.cand, H.stats, H.scal);
+            }
         }
         {
             ProfScope ps(H, "dg_score");
@@ -1855,9 +1886,17 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         }
         ProfScope ps(H, "dg_learned");
         const uint32_t audit = H.mlp_audit >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(H.mlp_audit * 4294967296.0);
-        hipLaunchKernelGGL((dg_learned_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                           H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.mlp_lo,
-                           H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.scal);
+        if (A.zsok) {
+            const size_t want = (ncell + 127) / 128, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 16;
+            hipLaunchKernelGGL((dg_learned_kernel<P, true>), dim3((unsigned)(want < cap ? want : cap)), dim3(128), 0,
+                               H.stream, Qout, H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2,
+                               H.g.gamma, H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist,
+                               H.scal);
+        } else {
+            hipLaunchKernelGGL((dg_learned_kernel<P, false>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout,
+                               H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma,
+                               H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist, H.scal);
+        }
     }
 }
 

@@ -343,7 +343,7 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         ZK_HIP(hipMalloc(&H->means, 4 * np * sizeof(double)));
         ZK_HIP(hipMalloc(&H->zsok, np));
         ZK_HIP(hipMemset(H->zsok, 0, np));
-        if (H->cfg.limiter == ZK_LIMITER_TVB) {
+        if (H->cfg.limiter == ZK_LIMITER_TVB || H->cfg.limiter == ZK_LIMITER_LEARNED) {
             ZK_HIP(hipMalloc(&H->tlist, (1 + nxu * nyu) * sizeof(int32_t)));
             ZK_HIP(hipMemset(H->tlist, 0, sizeof(int32_t)));
         }
@@ -462,10 +462,11 @@ zk_status zk_step(zk_handle *h, double dt, int nsteps)
     }
     ZK_HIP(hipSetDevice(H->cfg.device));
     hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, dt);
-    // WENO steps are short enough for launch overhead to matter on small grids:
-    // one step (dt is read from device memory) is captured once and replayed.
-    // DG stages are long, and the learned limiter salts its hashes per stage.
-    if (H->cfg.method == ZK_METHOD_WENO && H->use_graph && !H->prof && nsteps >= 2) {
+    // One step (dt is read from device memory) is captured once and replayed,
+    // which removes the per-launch gaps (a DG step is about 15 launches).  Not
+    // for the learned limiter or harvesting, which salt their hashes per stage.
+    const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
+    if (!salted && H->use_graph && !H->prof && nsteps >= 2) {
         if (!H->step_exec) {
             ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
             st = enqueue_step(H);
