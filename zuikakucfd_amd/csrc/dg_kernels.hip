@@ -145,7 +145,7 @@ DgTables make_tables(int P)
 __device__ __forceinline__ void euler_fg(const double U[4], double gam, double F[4], double G[4])
 {
     constexpr double eps = 1e-3;
-    const double rho = smax(U[0], eps), ir = 1.0 / rho;
+    const double rho = smax(U[0], eps), ir = rcp_nz(rho);
     const double u = U[1] * ir, v = U[2] * ir;
     const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
     F[0] = rho * u; F[1] = rho * u * u + p; F[2] = rho * u * v; F[3] = (U[3] + p) * u;

@@ -96,19 +96,20 @@ __global__ __launch_bounds__(zk::FV_THREADS) void ens_stage_kernel(EnsArgs E)
     }
 }
 
-// Cell positivity fallback, pass 2: a small grid-strided launch over the bad
-// cells pass 1 listed, (instance, cell); ends at once when there are none.
+// Cell positivity fallback, pass 2: a grid-strided launch over five work
+// items per bad cell pass 1 listed, (instance, cell); ends at once when there
+// are none.
 template <bool F32>
 __global__ __launch_bounds__(256) void ens_fallback_kernel(EnsArgs E)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) E.any[(E.stage + 1) % 3] = 0u;
-    const unsigned int n = E.any[E.stage];
+    const size_t n = (size_t)E.any[E.stage] * 5;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const int2 b = E.blist[t];
+        const int2 b = E.blist[t / 5];
         const EnsInst I = E.inst[b.x];
         FvArgs A = make_args(E, I, b.x);
         A.S = E.S + b.x;
-        zk::fv_fallback_bad<F32>(A, (size_t)b.y);
+        zk::fv_fallback_item<F32>(A, (size_t)b.y, (int)(t % 5));
     }
 }
 
@@ -258,7 +259,7 @@ void enqueue_ens_step(zk_ensemble *E)
             else hipLaunchKernelGGL((ens_stage_kernel<16, 15, false>), grid, blk, 0, E->stream, A);
         }
         if (A.bad) {
-            const dim3 fg(64);  // grid-strided over the listed bad cells
+            const dim3 fg(256);  // grid-strided over the listed bad cells
             if (f32) hipLaunchKernelGGL(ens_fallback_kernel<true>, fg, dim3(256), 0, E->stream, A);
             else hipLaunchKernelGGL(ens_fallback_kernel<false>, fg, dim3(256), 0, E->stream, A);
         }

@@ -5,7 +5,9 @@
 //   * FP64: the reference expressions with the divides merged.  The three
 //     nonlinear weights d_k/(eps+beta_k)^2 share one denominator, so a face
 //     side needs one divide instead of four; HLLC forms 1/rho and the Roe
-//     denominator once.  Results differ from the reference only by rounding.
+//     denominator once.  Divisors and radicands that are floored away from
+//     zero use div_nz / rcp_nz / sqrt_pos (zk_device.h).  Results differ from
+//     the reference only by rounding.
 //   * FP32 bulk: reconstruction on FP32 deltas relative to the face's left
 //     cell (WENO5 is translation-equivariant, so the FP32 error scales with
 //     the local jump rather than the magnitude), added back in FP64; pressure
@@ -14,6 +16,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "zk_device.h"
 
 namespace zk {
 
@@ -34,7 +38,7 @@ __device__ __forceinline__ void weno5_pair_f64(double UL2, double UL1, double UP
     const double s0 = t0 * t0, s1 = t1 * t1, s2 = t2 * t2;
     // alpha_k proportional to d_k * prod_{l != k} s_l
     const double a0 = 0.1 * (s1 * s2), a1 = 0.6 * (s0 * s2), a2 = 0.3 * (s0 * s1);
-    UL = (a0 * qL0 + a1 * qL1 + a2 * qL2) / (a0 + a1 + a2);
+    UL = div_nz(a0 * qL0 + a1 * qL1 + a2 * qL2, a0 + a1 + a2);  // a_k >= 0.1 eps^4
 
     const double qR0 = (11.0 / 6.0) * UR1 - (7.0 / 6.0) * UR2 + (1.0 / 3.0) * UR3;
     const double qR1 = qL2;
@@ -46,7 +50,7 @@ __device__ __forceinline__ void weno5_pair_f64(double UL2, double UL1, double UP
     const double u2 = eps + (C1 * d2 * d2 + C2 * r5 * r5);
     const double v0 = u0 * u0, v1 = u1 * u1, v2 = u2 * u2;
     const double b0 = 0.1 * (v1 * v2), b1 = 0.6 * (v0 * v2), b2 = 0.3 * (v0 * v1);
-    UR = (b0 * qR0 + b1 * qR1 + b2 * qR2) / (b0 + b1 + b2);
+    UR = div_nz(b0 * qR0 + b1 * qR1 + b2 * qR2, b0 + b1 + b2);
 }
 
 // HLLC (fluid.cpp:268-317), normal velocity = component 1.
@@ -54,19 +58,19 @@ __device__ __forceinline__ void hllc_fv_fast_f64(const double L[4], const double
 {
     constexpr double eps = 1e-12;
     const double gm1 = gamma - 1.0;
-    const double rhoL = smax(L[0], eps), irL = 1.0 / rhoL;
+    const double rhoL = smax(L[0], eps), irL = rcp_nz(rhoL);
     const double uL = L[1] * irL, vL = L[2] * irL, EL = L[3] * irL;
     const double pL = smax(gm1 * (L[3] - 0.5 * rhoL * (uL * uL + vL * vL)), eps);
-    const double aL = sqrt(gamma * pL * irL);
-    const double rhoR = smax(R[0], eps), irR = 1.0 / rhoR;
+    const double aL = sqrt_pos(gamma * pL * irL);
+    const double rhoR = smax(R[0], eps), irR = rcp_nz(rhoR);
     const double uR = R[1] * irR, vR = R[2] * irR, ER = R[3] * irR;
     const double pR = smax(gm1 * (R[3] - 0.5 * rhoR * (uR * uR + vR * vR)), eps);
-    const double aR = sqrt(gamma * pR * irR);
+    const double aR = sqrt_pos(gamma * pR * irR);
 
-    const double sqL = sqrt(rhoL), sqR = sqrt(rhoR), iden = 1.0 / (sqL + sqR);
+    const double sqL = sqrt_pos(rhoL), sqR = sqrt_pos(rhoR), iden = rcp_nz(sqL + sqR);
     const double u_roe = (sqL * uL + sqR * uR) * iden;
     const double h_roe = (sqL * (EL + pL * irL) + sqR * (ER + pR * irR)) * iden;
-    const double c_roe = sqrt(smax(gm1 * (h_roe - 0.5 * u_roe * u_roe), eps));
+    const double c_roe = sqrt_pos(smax(gm1 * (h_roe - 0.5 * u_roe * u_roe), eps));
     const double SL = smin(uL - aL, u_roe - c_roe);
     const double SR = smax(uR + aR, u_roe + c_roe);
 
@@ -82,14 +86,14 @@ __device__ __forceinline__ void hllc_fv_fast_f64(const double L[4], const double
     const double num = pR - pL + uL * mL - uR * mR;
     double den2 = mL - mR;
     if (fabs(den2) < 1e-12) den2 = (den2 >= 0 ? 1e-12 : -1e-12);
-    const double Sstar = num / den2;
+    const double Sstar = div_nz(num, den2);
 
     const bool left = Sstar >= 0.0;
     const double rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR;
     const double E = left ? EL : ER, p = left ? pL : pR, S = left ? SL : SR, m = left ? mL : mR;
     const double U0 = left ? L[0] : R[0], U1 = left ? L[1] : R[1], U2 = left ? L[2] : R[2],
                  U3 = left ? L[3] : R[3];
-    const double c = m / (S - Sstar);
+    const double c = m / (S - Sstar);  // this and p / m are unguarded: IEEE divisions
     F[0] = rho * u + S * (c - U0);
     F[1] = rho * u * u + p + S * (c * Sstar - U1);
     F[2] = rho * u * v + S * (c * v - U2);

@@ -47,18 +47,18 @@ __global__ __launch_bounds__(FV_THREADS) void fv_stage_kernel(FvArgs A)
     fv_tile_body<TX, TY, F32>(A, xcd_tile(blockIdx.x, A.ntiles), s_u, s_f);
 }
 
-// Cell positivity fallback, pass 2 (positivity option on): a small
-// grid-strided launch over the bad cells pass 1 listed, each with the good
-// neighbours it owns.  The counters rotate over the three stages: this launch
-// clears the next stage's.
+// Cell positivity fallback, pass 2 (positivity option on): a grid-strided
+// launch over five work items per bad cell pass 1 listed (the cell and the
+// good neighbours it owns).  The counters rotate over the three stages: this
+// launch clears the next stage's.
 template <bool F32>
 __global__ __launch_bounds__(256) void fv_fallback_kernel(FvArgs A, unsigned int *any, int slot)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) any[(slot + 1) % 3] = 0u;
     if (A.S->done) return;
-    const unsigned int n = any[slot];
+    const size_t n = (size_t)any[slot] * 5;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        fv_fallback_bad<F32>(A, (size_t)A.blist[t].y);
+        fv_fallback_item<F32>(A, (size_t)A.blist[t / 5].y, (int)(t % 5));
 }
 
 // max(|u| + c) over every cell (ghosts included), no floors:
@@ -124,7 +124,7 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     }
     if (A.bad && tiles) {
         ProfScope pf(H, "fv_fallback");
-        const dim3 grid(64);  // grid-strided over the (few) bad cells; ends at once when there are none
+        const dim3 grid(256);  // grid-strided over the (few) bad cells; ends at once when there are none
         if (f32) hipLaunchKernelGGL(fv_fallback_kernel<true>, grid, dim3(256), 0, H.stream, A, H.any, stage);
         else hipLaunchKernelGGL(fv_fallback_kernel<false>, grid, dim3(256), 0, H.stream, A, H.any, stage);
     }

@@ -230,11 +230,12 @@ __device__ __forceinline__ void fv_tile_body(const FvArgs &A, int tile, double (
     }
 }
 
-// Pass 2 driver for one bad cell k: the bad cell itself, then each good
-// updated neighbour whose lowest-index bad neighbour is k (so that every
-// affected cell is handled by exactly one thread, in a fixed way).
+// Pass 2 work item d of bad cell k: d = 4 is the bad cell itself, d = 0..3
+// its good updated neighbour N, W, E, S if k is that neighbour's
+// lowest-index bad neighbour (so that every affected cell is handled by
+// exactly one item, in a fixed way).  The items are independent.
 template <bool F32>
-__device__ __forceinline__ void fv_fallback_bad(const FvArgs &A, size_t k);
+__device__ __forceinline__ void fv_fallback_item(const FvArgs &A, size_t k, int d);
 
 // Cell positivity fallback, pass 2, for updated cell (i, j): a bad cell is
 // recomputed with first-order fluxes on its four faces; a good cell next to
@@ -324,26 +325,25 @@ __device__ __forceinline__ void fv_fallback_cell(const FvArgs &A, int i, int j)
 
 namespace zk {
 template <bool F32>
-__device__ __forceinline__ void fv_fallback_bad(const FvArgs &A, size_t k)
+__device__ __forceinline__ void fv_fallback_item(const FvArgs &A, size_t k, int d)
 {
     const int nx = A.nx;
-    const int i = (int)(k / nx), j = (int)(k % nx);
-    fv_fallback_cell<F32>(A, i, j);
-    const long off[4] = {-(long)nx, -1, 1, (long)nx};
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const size_t n = (size_t)((long)k + off[d]);
-        const int ni = (int)(n / nx), nj = (int)(n % nx);
-        if (ni < 3 || ni >= A.ny - 3 || nj < 3 || nj >= nx - 3) continue;  // outside the update range
-        if (A.bad[n] || !A.upd[n]) continue;
-        // lowest-index bad neighbour of n
-        size_t first = k;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const size_t m = (size_t)((long)n + off[e]);
-            if (A.bad[m] && m < first) first = m;
-        }
-        if (first == k) fv_fallback_cell<F32>(A, ni, nj);
+    if (d == 4) {
+        fv_fallback_cell<F32>(A, (int)(k / nx), (int)(k % nx));
+        return;
     }
+    const long off[4] = {-(long)nx, -1, 1, (long)nx};
+    const size_t n = (size_t)((long)k + off[d]);
+    const int ni = (int)(n / nx), nj = (int)(n % nx);
+    if (ni < 3 || ni >= A.ny - 3 || nj < 3 || nj >= nx - 3) return;  // outside the update range
+    if (A.bad[n] || !A.upd[n]) return;
+    // lowest-index bad neighbour of n
+    size_t first = k;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const size_t m = (size_t)((long)n + off[e]);
+        if (A.bad[m] && m < first) first = m;
+    }
+    if (first == k) fv_fallback_cell<F32>(A, ni, nj);
 }
 }  // namespace zk
