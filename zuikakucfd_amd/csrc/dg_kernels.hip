@@ -39,6 +39,22 @@ namespace {
 constexpr int DG_MAXNQ = 4;
 constexpr int DG_MAXNM = 16;
 
+// Row and column of update-range cell t (row-major over the nxu columns) in
+// 32-bit arithmetic: a grid holds fewer than 2^29 cells (zk_load checks it),
+// and a 64-bit division would cost a software routine per thread.
+__device__ __forceinline__ void interior_rc(size_t t, int nxu, int &ii, int &jj)
+{
+    const uint32_t tt = (uint32_t)t, q = tt / (uint32_t)nxu;
+    ii = (int)q;
+    jj = (int)(tt - q * (uint32_t)nxu);
+}
+__device__ __forceinline__ size_t interior_cell(size_t t, int nx, int nxu)
+{
+    int ii, jj;
+    interior_rc(t, nxu, ii, jj);
+    return (size_t)(ii + 3) * nx + (jj + 3);
+}
+
 // Basis tables, filled on the host in plain double arithmetic with the same
 // products the reference forms in phi2/dphi2 (src_DG/fluid.cpp:67-76).
 struct DgTables {
@@ -401,7 +417,8 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
     const DgTables &T = c_tab[P - 1];
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)A.nxu * A.nyu || A.S->done) return;
-    const int ii = (int)(t / A.nxu), jj = (int)(t % A.nxu);
+    int ii, jj;
+    interior_rc(t, A.nxu, ii, jj);
     const int i = ii + 3, j = jj + 3;
     const size_t k = (size_t)i * A.nx + j;
     const size_t np = A.np;
@@ -540,8 +557,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
     const int nxu = nx - 6, nyu = ny - 6;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nxu * nyu || S->done) return;
-    const int i = 3 + (int)(t / nxu), j = 3 + (int)(t % nxu);
-    const size_t k = (size_t)i * nx + j;
+    const size_t k = interior_cell(t, nx, nxu);
     double C[4][NM];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
@@ -614,25 +630,32 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
 }
 
 // Cells outside the update range: the stage combination with dU = 0, one
-// thread per (plane, ring cell) so that the launch fills the device.
+// thread per (plane, ring cell) so that the launch fills the device.  It also
+// clears the counts of the limiter lists filled after it (z0, z1, or null),
+// which saves two single-thread launches per stage.
 __global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__restrict__ Qin,
                                double *__restrict__ Qout, const int32_t *__restrict__ src,
                                const int32_t *__restrict__ ring, int nring, size_t np, int nm, double ca,
-                               double cb, int stage, const StepScalars *__restrict__ S, double *__restrict__ M)
+                               double cb, int stage, const StepScalars *__restrict__ S, double *__restrict__ M,
+                               int32_t *__restrict__ z0, int32_t *__restrict__ z1)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        if (z0) *z0 = 0;
+        if (z1) *z1 = 0;
+    }
     if (t >= (size_t)nring * 4 * nm || S->done) return;
-    const int p = (int)(t / nring);
-    const size_t k = (size_t)ring[t % nring];
+    const uint32_t tt = (uint32_t)t, p = tt / (uint32_t)nring;  // fewer than 2^32 (plane, ring cell) pairs
+    const size_t k = (size_t)ring[tt - p * (uint32_t)nring];
     const int32_t e = src[k];
     const size_t s = (size_t)(e >> kSrcShift);
-    const int var = p / nm;
+    const int var = (int)p / nm;
     const double qn = Qn[p * np + k];
     double qf = Qin[p * np + s];
     if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
     const double out = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
     Qout[p * np + k] = out;
-    if (M && p % nm == 0) M[(size_t)var * np + k] = out;
+    if (M && (int)p % nm == 0) M[(size_t)var * np + k] = out;
 }
 
 
@@ -692,7 +715,8 @@ __global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
     const size_t t = t0 + col;
     const bool live = t < ncell;
     const size_t tc = live ? t : ncell - 1;   // padded lanes compute on a valid cell, store nothing
-    const int ii = (int)(tc / A.nxu), jj = (int)(tc % A.nxu);
+    int ii, jj;
+    interior_rc(tc, A.nxu, ii, jj);
     const size_t k = (size_t)(ii + 3) * A.nx + (jj + 3);
     const size_t np = A.np;
     const bool upd = A.upd[k] != 0;
@@ -1430,7 +1454,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     if (t0 >= ncell) return;  // whole wave
     const bool live = t0 + col < ncell;
     const size_t t = live ? t0 + col : ncell - 1;
-    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+    const size_t k = interior_cell(t, nx, nxu);
     tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live);
 }
 
@@ -1470,7 +1494,7 @@ __global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict_
     bool need = false, is_cand = false;
     size_t k = 0;
     if (t < ncell) {
-        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+        k = interior_cell(t, nx, nxu);
         const unsigned z = zsok[k];
         is_cand = (z & 2) && upd[k];
         need = is_cand || !(z & 1);
@@ -1480,23 +1504,53 @@ __global__ __launch_bounds__(256) void dg_list_kernel(const uint8_t *__restrict_
     if (cand) wave_append(is_cand, cand, count, (int32_t)k);
 }
 
-// Full TVB + positivity work on the listed cells, 16 per wave, persistent grid.
+// Index of the want-th set bit (from 0) of mask, want < popcount(mask): the
+// largest p with at most want set bits below p, by binary search.
+__device__ __forceinline__ int nth_set_bit(unsigned long long mask, int want)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1)
+        if (__popcll(mask & ((1ull << (pos + w)) - 1ull)) <= want) pos += w;
+    return pos;
+}
+
+// TVB limiter after the fused stage, without a list: a wave scans 64 cells
+// of the update range (one per lane) from the byte the fused kernel left per
+// cell, gives flag 0 to those that need nothing (certified and not an updated
+// candidate), and treats the others itself, 16 at a time, with the
+// matrix-core TVB + positivity routine.  Limiting changes no mean, so the
+// cells are independent and the order does not matter.
 template <int P>
-__global__ __launch_bounds__(256) void dg_tvb_list_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+__global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
-                                                          const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
-                                                          int nx, size_t np, double Mh2, double gamma,
-                                                          const int32_t *__restrict__ list,
+                                                          const int32_t *__restrict__ bct,
+                                                          const uint8_t *__restrict__ upd,
+                                                          uint8_t *__restrict__ flags,
+                                                          const uint8_t *__restrict__ zsok, int nx, int nxu,
+                                                          size_t ncell, size_t np, double Mh2, double gamma,
                                                           const StepScalars *__restrict__ S)
 {
-    if (S->done) return;
-    const int n = list[0];
-    const int col = threadIdx.x & 15;
-    const int waves = gridDim.x * (blockDim.x >> 6);
-    for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16; base < n; base += waves * 16) {
-        const bool live = base + col < n;
-        const size_t k = (size_t)list[1 + (live ? base + col : n - 1)];
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, k, live);
+    if (S->done) return;  // uniform over the grid
+    const int lane = threadIdx.x & 63, col = lane & 15;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    if (t0 >= ncell) return;  // whole wave
+    const size_t t = t0 + lane;
+    bool need = false;
+    size_t k = 0;
+    if (t < ncell) {
+        k = interior_cell(t, nx, nxu);
+        const unsigned z = zsok[k];
+        need = ((z & 2) && upd[k]) || !(z & 1);
+        if (!need) flags[k] = 0;
+    }
+    const unsigned long long mask = __ballot(need);
+    const int n = __popcll(mask);
+    for (int g0 = 0; g0 < n; g0 += 16) {
+        // column col of this group takes listed cell g0 + col (padding repeats the last one)
+        const int want = g0 + col < n ? g0 + col : n - 1;
+        const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n);
     }
 }
 
@@ -1544,7 +1598,7 @@ __global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__
     bool is = false;
     size_t k = 0;
     if (t < ncell) {
-        k = (size_t)(t / nxu + 3) * nx + (t % nxu + 3);
+        k = interior_cell(t, nx, nxu);
         if (bct[k] == 0) {
             double C[4][NM];
 #pragma unroll
@@ -1693,7 +1747,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
     const int nxu = nx - 6, nyu = ny - 6;
     const size_t n = LIST ? (size_t)list[0] : (size_t)nxu * nyu;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = LIST ? (size_t)list[1 + t] : (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+        const size_t k = LIST ? (size_t)list[1 + t] : interior_cell(t, nx, nxu);
         learned_cell<P>(Q, M, src, bct, flags, score, nx, np, Mh2, gamma, lo, hi, audit_thresh, salt, stats,
                         LIST ? nullptr : zsok, k);
     }
@@ -1715,7 +1769,7 @@ __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restric
     const int nxu = nx - 6, nyu = ny - 6;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nxu * nyu || S->done) return;
-    const size_t k = (size_t)(3 + t / nxu) * nx + (3 + t % nxu);
+    const size_t k = interior_cell(t, nx, nxu);
     if (bct[k] != 0) return;
     double C[4][NM];
 #pragma unroll
@@ -1826,11 +1880,20 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         else
             hipLaunchKernelGGL((dg_cell_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, A);
     }
+    // list counts the limiter pass below fills: cleared by the ring kernel
+    int32_t *z0 = nullptr, *z1 = nullptr;
+    if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
+        z0 = H.cand;
+        if (A.zsok) z1 = H.tlist;
+    }
     if (H.nring > 0) {
         ProfScope ps(H, "dg_ring");
         const size_t nt = (size_t)H.nring * 4 * H.nm;
         hipLaunchKernelGGL(dg_ring_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, H.stream, Qn, Qin, Qout,
-                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means);
+                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means, z0, z1);
+    } else {
+        if (z0) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, z0);
+        if (z1) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, z1);
     }
     const double Mh2 = A.Mh2;
     H.stage_counter += 1;
@@ -1847,15 +1910,10 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // after the fused stage (zsok written): a per-thread screen lists the cells that
         // are limited or not certified, and the matrix-core kernel treats only those;
         // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
-        if (A.zsok && H.tlist && !H.no_mfma) {
-            hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
-            hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream, A.zsok,
-                               H.upd, H.flags, H.g.nx, A.nxu, ncell, H.tlist, (int32_t *)nullptr,
-                               (unsigned long long *)nullptr, H.scal);
-            const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
-            hipLaunchKernelGGL((dg_tvb_list_kernel<P>), dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
-                               H.stream, Qout, H.means, H.src, H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma,
-                               H.tlist, H.scal);
+        if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
+            hipLaunchKernelGGL((dg_tvb_scan_kernel<P>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream,
+                               Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
+                               H.g.gamma, H.scal);
         } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
                                H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
@@ -1867,9 +1925,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {
             ProfScope ps(H, "dg_candidates");
-            hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.cand);
+            // list counts cleared with the ring
             if (A.zsok) {  // the fused kernel marked the candidates: list them and the uncertified cells
-                hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, H.tlist);
                 hipLaunchKernelGGL(dg_list_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream,
                                    A.zsok, H.upd, H.flags, H.g.nx, A.nxu, ncell, H.tlist, H.cand, H.stats + 6, H.scal);
             } else {

@@ -343,7 +343,7 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         ZK_HIP(hipMalloc(&H->means, 4 * np * sizeof(double)));
         ZK_HIP(hipMalloc(&H->zsok, np));
         ZK_HIP(hipMemset(H->zsok, 0, np));
-        if (H->cfg.limiter == ZK_LIMITER_TVB || H->cfg.limiter == ZK_LIMITER_LEARNED) {
+        if (H->cfg.limiter == ZK_LIMITER_LEARNED) {
             ZK_HIP(hipMalloc(&H->tlist, (1 + nxu * nyu) * sizeof(int32_t)));
             ZK_HIP(hipMemset(H->tlist, 0, sizeof(int32_t)));
         }

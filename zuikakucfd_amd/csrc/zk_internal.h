@@ -84,7 +84,7 @@ struct Handle {
     unsigned int *any = nullptr;  // bad-cell counts, one per stage
     int2 *blist = nullptr;        // the bad cells of the current stage
     uint8_t *zsok = nullptr;      // DG TVB: cells whose positivity scaling is certified a no-op
-    int32_t *tlist = nullptr;     // DG TVB: [0] = count, then the cells the screen passes on
+    int32_t *tlist = nullptr;     // DG learned: [0] = count, then the cells the limiter pass treats
     int ncu = 0;              // compute units of the device
 
     // kernel timing: when prof is set every launcher brackets its kernel with
