@@ -5,6 +5,8 @@
 * ``bin/zk_weno``, ``bin/zk_dg{2,3,4}`` -- the drop-in host executables
                     (C++17, g++) that read params.txt/rho,u,v,p,bctype.dat and
                     drive libzk.so with the reference's command line.
+* ``tests/_bin/*_check`` -- device-arithmetic check programs the GPU tests
+                    run (from ``tests/*_check.hip``).
 * the oracle     -- ``oracle/liboracle.so`` via ``make -C oracle`` (test
                     infrastructure; built here so a checkout has one entry).
 
@@ -88,6 +90,18 @@ def build_bins() -> list[Path]:
     return outs
 
 
+def build_checks() -> list[Path]:
+    """Device-arithmetic check programs that the GPU tests run (tests/_bin/)."""
+    out_dir = ROOT / "tests" / "_bin"
+    out_dir.mkdir(exist_ok=True)
+    outs = []
+    for src in sorted((ROOT / "tests").glob("*_check.hip")):
+        out = out_dir / src.stem
+        _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", str(src), "-o", str(out)])
+        outs.append(out)
+    return outs
+
+
 def build_oracle() -> Path:
     make = shutil.which("make") or "make"
     _run([make, "-s", "-C", str(ROOT / "oracle"), "-B", "liboracle.so"])
@@ -97,6 +111,7 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_lib()
     build_bins()
+    build_checks()
     build_oracle()
 
 
