@@ -176,14 +176,16 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
  * fill whole rounds of the device), 10 + R = fused single-wave form,
  * 2/3/4 = split kernels with the P=3 MFMA cell kernel at that minimum of
  * waves per SIMD, -1 = split kernels with the VALU cell kernel; use_graph = 1
- * (default) lets zk_step replay a captured WENO step as a HIP graph. */
+ * (default) lets zk_step replay a captured step as a HIP graph (WENO, and DG
+ * unless the limiter is learned or harvesting is on). */
 zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph);
 
 /* ---------------------------------------------------------------------------
  * Learned troubled-cell indicator for DG P=2 (no reference counterpart).
  * A 192-64-64-1 MLP (BF16 weights on v_mfma_f32_16x16x32_bf16, FP32
- * accumulation, FP32 Adam master weights) scores every cell from its own and
- * its four neighbours' normalised modes.  Training data are (features, TVB
+ * accumulation, FP32 Adam master weights) scores a cell from its own and its
+ * four neighbours' normalised modes; the limiter asks it only about the TVB
+ * candidates (cells with a linear mode above M h^2).  Training data are (features, TVB
  * flag) pairs harvested from the solver's own TVB runs on the device.
  * ------------------------------------------------------------------------- */
 typedef struct zk_mlp zk_mlp;
