@@ -225,6 +225,41 @@ def test_dg_random_mask(P, variant, nx):
     np.testing.assert_array_equal(s.get_flags(), o.flags)
 
 
+@pytest.mark.parametrize("P", [2, 3])
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3"])
+def test_dg_fused_memory_modes(P, mode, monkeypatch):
+    """The fused kernel's memory pipelines (ZK_DMA, read when the handle is
+    created): tables from constant memory (0), tables in LDS (1), rows by
+    global-to-LDS loads into one (2) or two (3, the default) row buffers.  The
+    irregular mask at nx = 150 gives three 62-column runs, the last partial,
+    and 4-row segments put segment edges everywhere."""
+    say(f"dg P={P} fused memory mode {mode}")
+    monkeypatch.setenv("ZK_DMA", mode)
+    rng = np.random.default_rng(31 + P)
+    ny, nx = 41, 150
+    m = meshio.riemann_2d(64)
+    m.rho = 0.5 + rng.random((ny, nx)); m.u = rng.uniform(-1, 1, (ny, nx))
+    m.v = rng.uniform(-1, 1, (ny, nx)); m.p = 0.5 + rng.random((ny, nx))
+    b = meshio.ring_bctype(ny, nx)
+    b[10:13, 20:25] = -1
+    b[5:7, 40] = 1
+    b[20:24, 70] = -1
+    b[8, 120:125] = 1
+    m.bctype = b
+    m.nx, m.ny = nx, ny
+    s, o = dg_pair(m, P, zk.LIMITER_TVB, M=50.0)
+    monkeypatch.delenv("ZK_DMA")
+    dt = 0.5 * oracle.dg_cfl_limit(P) * m.da / o.max_speed()
+    for variant in (104, 100):
+        s.set_tuning(0, variant)
+        s.step(dt, 3)
+        o.step(dt, 3)
+        Q = s.get_state()
+        for c in range(4):
+            assert rel_l1(Q[c], o.Q[c]) <= TOL, (variant, c)
+        np.testing.assert_array_equal(s.get_flags(), o.flags)
+
+
 @pytest.mark.parametrize("P", [1, 2, 3])
 def test_dg_smooth_periodic_flags(P):
     """Smooth periodic wave with L2-projected modes: TVB flags identical (all
