@@ -104,6 +104,36 @@ double zsP_h(int p, double x)
     return 0.0;
 }
 
+// Legendre values, derivatives and Gauss-Legendre nodes/weights as constant
+// expressions, so that device code with unrolled indices folds them into
+// instruction operands (the fused kernel's 4x4-block form).  Same literals
+// and closed forms as legP_h / legD_h and the tables below.
+__host__ __device__ constexpr double leg_c(int n, double x)
+{
+    return n == 0 ? 1.0 : n == 1 ? x : n == 2 ? 0.5 * (3.0 * x * x - 1.0) : n == 3 ? 0.5 * (5.0 * x * x * x - 3.0 * x) : 0.0;
+}
+__host__ __device__ constexpr double legd_c(int n, double x)
+{
+    return n == 0 ? 0.0 : n == 1 ? 1.0 : n == 2 ? 3.0 * x : n == 3 ? 0.5 * (15.0 * x * x - 3.0) : 0.0;
+}
+__host__ __device__ constexpr double gl_x(int nq, int q)
+{
+    return nq == 2   ? (q == 0 ? -0.5773502691896257 : 0.5773502691896257)
+           : nq == 3 ? (q == 0 ? -0.7745966692414834 : q == 1 ? 0.0 : 0.7745966692414834)
+           : nq == 4 ? (q == 0   ? -0.8611363115940526
+                        : q == 1 ? -0.3399810435848563
+                        : q == 2 ? 0.3399810435848563
+                                 : 0.8611363115940526)
+                     : 0.0;
+}
+__host__ __device__ constexpr double gl_w(int nq, int q)
+{
+    return nq == 2   ? 1.0
+           : nq == 3 ? (q == 1 ? 0.8888888888888888 : 0.5555555555555556)
+           : nq == 4 ? ((q == 0 || q == 3) ? 0.3478548451374538 : 0.6521451548625461)
+                     : 0.0;
+}
+
 // Gauss-Legendre literals of src_DG/fluid.cpp:94-111.
 const double GLP_H[5][4] = {{0, 0, 0, 0},
                             {0.0, 0, 0, 0},
@@ -823,6 +853,53 @@ struct MfmaTabsF {
 };
 __constant__ MfmaTabsF c_mff[3];
 
+MfmaTabsF make_mfma_tables_fused(int P);
+
+// Tables of the fused kernel's 4x4-block form (F4), for v_mfma_f64_4x4x4f64:
+// four independent 4x4x4 products per instruction, D_b = A_b B_b + C_b for
+// block b = (lane >> 2) & 3.  Lane 16 r + 4 b + c holds A_b[c][r], B_b[r][c]
+// and D_b[r][c] (probed on gfx950).  With B = Q[c][v], the lane (g, col)
+// layout of the fused kernel (modes (px = v, py = g) of cell col), block b
+// holds cells 4b..4b+3 and D_b[i][col & 3] lands in lane (i, col): a
+// constant A contracts over py (the lane group) for 4 cells at once, so the
+// tensor-product (sum-factorised) steps need no data movement.  A operands
+// are per-lane values (lane -> A[c][r]), the same for every block.
+struct MfmaTabs4 {
+    double aL[64];      // A[b][py] = L_py(eta_b): values at the Gauss rows eta_b
+    double aNv[4][64];  // A[i][py] = L_v(xi_i) L_py(-1): N trace at xi_i, the px = v term
+    double aSv[4][64];  // A[i][py] = L_v(xi_i) L_py(+1): S trace
+    double aGN[4][64];  // A[i][q] = w_q L_v(xi_q) for every i: N-face projection onto px = v
+    double aWL[64];     // A[py][q] = w_q L_py(eta_q): x-face and volume F projection onto py
+    double aWD[64];     // A[py][q] = w_q L'_py(eta_q): volume G projection onto py
+    double zc[16];      // as MfmaTabsF::zc
+};
+__constant__ MfmaTabs4 c_mf4[3];
+
+MfmaTabs4 make_mfma_tables4(int P)
+{
+    MfmaTabs4 M{};
+    const int NQ = P + 1;
+    const double *gp = GLP_H[NQ], *gw = GLW_H[NQ];
+    auto loc = [&](int q) { return q < NQ ? gp[q] : 0.0; };
+    auto wt = [&](int q) { return q < NQ ? gw[q] : 0.0; };
+    auto L = [&](int n, double x) { return n <= P ? legP_h(n, x) : 0.0; };
+    auto D = [&](int n, double x) { return n <= P ? legD_h(n, x) : 0.0; };
+    for (int l = 0; l < 64; ++l) {
+        const int r = l >> 4, c = l & 3;  // A[c][r] sits in this lane
+        M.aL[l] = c < NQ ? L(r, loc(c)) : 0.0;
+        for (int v = 0; v < 4; ++v) {
+            M.aNv[v][l] = c < NQ ? L(v, loc(c)) * L(r, -1.0) : 0.0;
+            M.aSv[v][l] = c < NQ ? L(v, loc(c)) * L(r, +1.0) : 0.0;
+            M.aGN[v][l] = wt(r) * L(v, loc(r));
+        }
+        M.aWL[l] = wt(r) * L(c, loc(r));
+        M.aWD[l] = wt(r) * D(c, loc(r));
+    }
+    const MfmaTabsF F = make_mfma_tables_fused(P);
+    for (int i = 0; i < 16; ++i) M.zc[i] = F.zc[i];
+    return M;
+}
+
 MfmaTabsF make_mfma_tables_fused(int P)
 {
     MfmaTabsF M{};
@@ -898,15 +975,23 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // variable w), so the next row is in flight while this one is computed,
 // without holding registers for it; 3 is 2 with two row buffers, which
 // saves the barrier that frees the single buffer.
-template <int P, int MINW, bool WG, int MODE>
+//
+// F4: the products run as 4x4x4 blocks (v_mfma_f64_4x4x4f64, MfmaTabs4) on
+// the sum-factorised form: values are contracted over py (the lane group) by
+// the matrix core, four cells per block, and over px (the register index) by
+// VALU multiply-adds.  Same lane layout, same results up to summation order.
+template <int P, int MINW, bool WG, int MODE, bool F4>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     constexpr bool DMA = MODE >= 2, DB = MODE == 3;
     constexpr int NB = DB ? 2 : 1, NS = DB ? 4 : 2;  // row buffers, ghost-map row slots
     static_assert(WG || MODE == 0, "LDS tables need the workgroup form");
-    __shared__ double tabs_lds[MODE >= 1 ? sizeof(MfmaTabsF) / 8 : 1];
+    static_assert(!F4 || MODE >= 1, "the 4x4-block form reads its tables from LDS");
+    constexpr size_t TAB_BYTES = F4 ? sizeof(MfmaTabs4) : sizeof(MfmaTabsF);
+    __shared__ double tabs_lds[MODE >= 1 ? TAB_BYTES / 8 : 1];
     const MfmaTabsF &M = (MODE >= 1) ? *reinterpret_cast<const MfmaTabsF *>(tabs_lds) : c_mff[P - 1];
+    const MfmaTabs4 &M4 = *reinterpret_cast<const MfmaTabs4 *>(tabs_lds);
     const DgTables &T = c_tab[P - 1];
     __shared__ double lds_w[WG ? 4 : 1][4][4];   // [wave][g][c]: W trace of the wave's column 0
     __shared__ double lds_ge[WG ? 4 : 1][4][4];  // [wave][g][c]: projected flux of its column-15 face
@@ -935,10 +1020,13 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         store_col = col >= 1 && col <= 14 && j <= A.nx - 4;
     }
     if (MODE >= 1) {
-        const double *src = reinterpret_cast<const double *>(&c_mff[P - 1]);
-        for (int i = threadIdx.x; i < (int)(sizeof(MfmaTabsF) / 8); i += 256) tabs_lds[i] = src[i];
+        const double *src = F4 ? reinterpret_cast<const double *>(&c_mf4[P - 1])
+                               : reinterpret_cast<const double *>(&c_mff[P - 1]);
+        for (int i = threadIdx.x; i < (int)(TAB_BYTES / 8); i += 256) tabs_lds[i] = src[i];
         __syncthreads();
     }
+    auto mf4 = [](double a, double b) { return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, 0.0, 0, 0, 0); };
+    auto mf4c = [](double a, double b, double c) { return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0); };
     const int jc = j < A.nx ? j : A.nx - 1;           // clamped for loads past the right edge
     const int r0 = 3 + seg * R;
     const int r1 = (r0 + R < A.ny - 3) ? r0 + R : A.ny - 3;  // rows [r0, r1), halo rows r0-1 and r1
@@ -1023,10 +1111,18 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         load(r0 - 1, Q);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            d4 t = {0.0, 0.0, 0.0, 0.0};
+            if constexpr (F4) {
+                double t = 0.0;
 #pragma unroll
-            for (int s = 0; s <= P; ++s) t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][lane], Q[c][s], t, 0, 0, 0);
-            Sup[c] = t[3];
+                for (int v = 0; v <= P; ++v) t = mf4c(M4.aSv[v][lane], Q[c][v], t);
+                Sup[c] = t;
+            } else {
+                d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s <= P; ++s)
+                    t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][lane], Q[c][s], t, 0, 0, 0);
+                Sup[c] = t[3];
+            }
         }
     }
     // pending row r-1: out = base - fs * gS, its S-face projection gS still to come
@@ -1052,6 +1148,28 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         asm volatile("" : "+s"(npr));
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
+        // With two row buffers, row r stays in LDS for the whole iteration, so
+        // at P = 3 its modes are read again where they are needed instead of
+        // being held in 32 registers across the row (through a fresh opaque
+        // lane index, so the reads are not merged with the first one).
+        auto reread = [&](double Qr[4][4]) {
+            int l2 = lane;
+            asm volatile("" : "+v"(l2));
+            const int x = 16 * wvu + (l2 & 15);
+            const int32_t eo = srcbuf[r % NS][x];
+            auto qb = qbuf[DB ? (r & 1) : 0];
+            const int gq = (l2 >> 4) <= P ? (l2 >> 4) : 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Qr[c][v] = (v <= P && slot) ? qb[c * NM + v * NQ + gq][x] : 0.0;
+            if (eo & 1)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Qr[1][v] = -Qr[1][v];
+            if (eo & 2)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Qr[2][v] = -Qr[2][v];
+        };
         double Q[4][4];
         if (DMA) {
             // row r is complete in qbuf once every wave's loads have landed
@@ -1089,10 +1207,26 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         d4 Tr[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            d4 t = {0.0, 0.0, 0.0, 0.0};
+            if constexpr (F4) {
+                // W, E: the modes at eta_g (contracted over py), then L_v(-+1) over v;
+                // N, S: L_v(xi_g) L_py(-+1) accumulated over v on the matrix core
+                double w = 0.0, e = 0.0, n = 0.0, sv = 0.0;
 #pragma unroll
-            for (int s = 0; s <= P; ++s) t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][ln], Q[c][s], t, 0, 0, 0);
-            Tr[c] = t;
+                for (int v = 0; v <= P; ++v) {
+                    const double t = mf4(M4.aL[ln], Q[c][v]);
+                    w += (v & 1) ? -t : t;
+                    e += t;
+                    n = mf4c(M4.aNv[v][ln], Q[c][v], n);
+                    sv = mf4c(M4.aSv[v][ln], Q[c][v], sv);
+                }
+                Tr[c] = d4{w, e, n, sv};
+            } else {
+                d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s <= P; ++s)
+                    t = __builtin_amdgcn_mfma_f64_16x16x4f64(M.trA[s][ln], Q[c][s], t, 0, 0, 0);
+                Tr[c] = t;
+            }
         }
         // y face between row r-1 (S trace Sup) and row r (N trace), projected onto px = v
         d4 gN[4];
@@ -1104,8 +1238,14 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             else hllc_dg_fast_f64(a, b, gam, Gs);
             const double G[4] = {Gs[0], Gs[2], Gs[1], Gs[3]};
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                gN[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(M.ayA[ln], G[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+            for (int c = 0; c < 4; ++c) {
+                if constexpr (F4) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) gN[c][v] = v <= P ? mf4(M4.aGN[v][ln], G[c]) : 0.0;
+                } else {
+                    gN[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(M.ayA[ln], G[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                }
+            }
         }
         // sab[c]: sum over the non-mean modes of |mode| * max |basis| at the positivity points
         double sab[4] = {0.0, 0.0, 0.0, 0.0}, mean0[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1122,7 +1262,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                         mean0[c] = out;
                         if (A.means) A.means[(size_t)c * np + kp] = out;
                     } else {
-                        sab[c] += fabs(out) * M.zc[4 * v + g];
+                        sab[c] += fabs(out) * (F4 ? M4.zc[4 * v + g] : M.zc[4 * v + g]);
                     }
                     if ((v == 1 && g == 0) || (v == 0 && g == 1)) lin_big = lin_big || !(fabs(out) <= A.Mh2);
                 }
@@ -1191,8 +1331,13 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             double gE[4], gW[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const d4 ge = __builtin_amdgcn_mfma_f64_16x16x4f64(M.axA[ln], Fx[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
-                gE[c] = ge[0];                             // py = g projection of face (col | col+1)
+                if constexpr (F4) {
+                    gE[c] = mf4(M4.aWL[ln], Fx[c]);      // py = g projection of face (col | col+1)
+                } else {
+                    const d4 ge =
+                        __builtin_amdgcn_mfma_f64_16x16x4f64(M.axA[ln], Fx[c], d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                    gE[c] = ge[0];                         // py = g projection of face (col | col+1)
+                }
                 gW[c] = __shfl_up(gE[c], 1, 16);           // face (col-1 | col)
             }
             if (WG) {
@@ -1209,7 +1354,43 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[c][v] += ((v & 1) ? -gW[c] : gW[c]) - gE[c];  // L_v(-1) gW - gE
         }
-        {
+        if constexpr (F4) {
+            // the coefficients L_v(xi_a), w_a L'_v(xi_a), w_a L_v(xi_a) fold to
+            // constants (instruction operands) in the unrolled loops
+            if (DB && P == 3) reread(Q);
+            // the state at the points (xi_a, eta_g): modes at eta_g (contracted
+            // over py on the matrix core), then sum_v L_v(xi_a) over px
+            double U[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double t[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) t[v] = v <= P ? mf4(M4.aL[ln], Q[c][v]) : 0.0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    double u = 0.0;
+#pragma unroll
+                    for (int v = 0; v <= P; ++v) u += (a < NQ ? leg_c(v, gl_x(NQ, a)) : 0.0) * t[v];
+                    U[c][a] = u;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a <= P; ++a) {
+                // fluxes at (xi_a, eta_g) and their projections: over eta on the
+                // matrix core, then w_a L'_px(xi_a) / w_a L_px(xi_a) per px
+                const double Up[4] = {U[0][a], U[1][a], U[2][a], U[3][a]};
+                double F[4], G[4];
+                euler_fg(Up, gam, F, G);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double zf = mf4(M4.aWL[ln], F[c]), zg = mf4(M4.aWD[ln], G[c]);
+#pragma unroll
+                    for (int v = 0; v <= P; ++v)
+                        acc[c][v] += (gl_w(NQ, a) * legd_c(v, gl_x(NQ, a))) * zf + (gl_w(NQ, a) * leg_c(v, gl_x(NQ, a))) * zg;
+                }
+            }
+        } else {
+            if (DB && P == 3) reread(Q);
             d4 U[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -1236,6 +1417,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const double cdt = (A.stage == 0) ? dt : A.cb * dt;
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
+        if (DB && P == 3) reread(Q);
         if (store_col && slot) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -1844,20 +2026,23 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
         if (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
-        if (wg && H.dma == 3)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
-                               nrun);
+        if (wg && H.dma == 3 && H.mf4)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3, true>), dim3((unsigned)units), dim3(256), 0, H.stream,
+                               A, R, nrun);
+        else if (wg && H.dma == 3)
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
+                               A, R, nrun);
         else if (wg && H.dma == 2)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 2>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
-                               nrun);
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 2, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
+                               A, R, nrun);
         else if (wg && H.dma == 1)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 1>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
-                               nrun);
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 1, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
+                               A, R, nrun);
         else if (wg)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 0>), dim3((unsigned)units), dim3(256), 0, H.stream, A, R,
-                               nrun);
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 0, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
+                               A, R, nrun);
         else
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false, 0>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
+            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false, 0, false>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
                                H.stream, A, R, nrun);
     } else if (ncell > 0) {
         {
@@ -1974,6 +2159,9 @@ hipError_t dg_upload_tables(int device)
     if (e != hipSuccess) return e;
     const MfmaTabsF F[3] = {make_mfma_tables_fused(1), make_mfma_tables_fused(2), make_mfma_tables_fused(3)};
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_mff), F, sizeof(F), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    const MfmaTabs4 F4[3] = {make_mfma_tables4(1), make_mfma_tables4(2), make_mfma_tables4(3)};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_mf4), F4, sizeof(F4), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     if (track) g_tables_ready[device] = true;
     return hipSuccess;

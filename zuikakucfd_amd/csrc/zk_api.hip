@@ -226,9 +226,11 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     H->cfg = *cfg;
     H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
     // measurement overrides: ZK_NO_MFMA=1 (VALU cell kernel), ZK_DMA=0..3 (memory pipeline of
-    // the fused DG kernel's workgroup form), ZK_TUNE=<dg_variant> (a value zk_set_tuning
-    // accepts; anything else is ignored)
+    // the fused DG kernel's workgroup form), ZK_MF4=0 (its 16x16x4 products instead of the
+    // 4x4x4 blocks), ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is
+    // ignored)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
+    if (const char *v = std::getenv("ZK_MF4")) H->mf4 = v[0] != '0';
     if (const char *v = std::getenv("ZK_DMA"))
         if (v[0] >= '0' && v[0] <= '3' && v[1] == 0) H->dma = v[0] - '0';
     if (const char *v = std::getenv("ZK_TUNE")) {

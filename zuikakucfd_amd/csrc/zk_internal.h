@@ -48,6 +48,7 @@ struct Handle {
     int dma = 3;              // workgroup fused DG kernel: 0 tables from constant memory, 1 tables in LDS,
                               // 2 tables in LDS and rows by global-to-LDS loads, 3 the same with two
                               // row buffers (ZK_DMA overrides)
+    bool mf4 = true;          // fused DG kernel (two row buffers) on 4x4x4 matrix-core blocks (ZK_MF4=0: 16x16x4)
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
     bool use_graph = true;    // WENO zk_step: replay one captured step as a HIP graph
     hipGraph_t step_graph = nullptr;
