@@ -170,8 +170,8 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
 
 /* Kernel-variant selection for the autotuner (no reference counterpart):
  * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
- * automatic (fused row-marching stage kernel, workgroup form, at P >= 2
- * without the Persson limiter; split face/cell kernels otherwise), 100 + R =
+ * automatic (fused row-marching stage kernel, workgroup form, without the
+ * Persson limiter; split face/cell kernels otherwise), 100 + R =
  * fused workgroup form with R-row segments (R in 4..64; 100 = R chosen to
  * fill whole rounds of the device), 10 + R = fused single-wave form,
  * 2/3/4 = split kernels with the P=3 MFMA cell kernel at that minimum of
