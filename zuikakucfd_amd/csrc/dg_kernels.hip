@@ -18,6 +18,7 @@
 // a per-thread screen lists the cells that are limited or not certified, and a
 // matrix-core kernel limits and applies the positivity scaling to those.
 // The learned indicator (P = 2) scores only the TVB candidate cells.
+#include <atomic>
 #include <mutex>
 
 #include "zk_internal.h"
@@ -1149,9 +1150,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
         // With two row buffers, row r stays in LDS for the whole iteration, so
-        // at P = 3 its modes are read again where they are needed instead of
-        // being held in 32 registers across the row (through a fresh opaque
-        // lane index, so the reads are not merged with the first one).
+        // its modes are read again where they are needed instead of being held
+        // in registers across the row (through a fresh opaque lane index, so
+        // the reads are not merged with the first one).  At P = 3 this is what
+        // keeps the kernel within 256 VGPRs; at P = 1 it makes room for a third
+        // wave per SIMD; at P = 2 holding the modes is faster.
         auto reread = [&](double Qr[4][4]) {
             int l2 = lane;
             asm volatile("" : "+v"(l2));
@@ -1357,7 +1360,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         if constexpr (F4) {
             // the coefficients L_v(xi_a), w_a L'_v(xi_a), w_a L_v(xi_a) fold to
             // constants (instruction operands) in the unrolled loops
-            if (DB && P == 3) reread(Q);
+            if (DB && P != 2) reread(Q);
             // the state at the points (xi_a, eta_g): modes at eta_g (contracted
             // over py on the matrix core), then sum_v L_v(xi_a) over px
             double U[4][4];
@@ -1390,7 +1393,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 }
             }
         } else {
-            if (DB && P == 3) reread(Q);
+            if (DB && P != 2) reread(Q);
             d4 U[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -1417,7 +1420,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const double cdt = (A.stage == 0) ? dt : A.cb * dt;
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
-        if (DB && P == 3) reread(Q);
+        if (DB && P != 2) reread(Q);
         if (store_col && slot) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -2011,11 +2014,26 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         int R = H.tune >= 100 ? H.tune - 100 : (H.tune >= 10 ? H.tune - 10 : 0);
         const int width = wg ? 62 : 14;                 // updated columns per wave (per workgroup for wg)
         const int nrun = (A.nxu + width - 1) / width;
+        // the kernel instance this stage launches
+        const void *kern = (wg && H.dma == 3 && H.mf4) ? (const void *)dg_fused_kernel<P, 2, true, 3, true>
+                           : (wg && H.dma == 3)         ? (const void *)dg_fused_kernel<P, 2, true, 3, false>
+                           : (wg && H.dma == 2)         ? (const void *)dg_fused_kernel<P, 2, true, 2, false>
+                           : (wg && H.dma == 1)         ? (const void *)dg_fused_kernel<P, 2, true, 1, false>
+                           : wg                         ? (const void *)dg_fused_kernel<P, 2, true, 0, false>
+                                                        : (const void *)dg_fused_kernel<P, 2, false, 0, false>;
         if (R <= 0) {
-            // Whole rounds of the device: at 2 waves per SIMD a CU holds 2
-            // workgroups (or 8 single waves).  The fewest rounds with R <= 64
-            // keep halo rows few and leave no part-filled last round.
-            const long cap = (long)(H.ncu > 0 ? H.ncu : 256) * (wg ? 2 : 8);
+            // Whole rounds of the device: as many workgroups (or single waves)
+            // as the CUs hold at the kernel's occupancy.  The fewest rounds with
+            // R <= 64 keep halo rows few and leave no part-filled last round.
+            static std::atomic<int> occ_cache[6];  // per kernel instance of this P (0 = not known yet)
+            const int slot_i = !wg ? 5 : H.dma == 3 ? (H.mf4 ? 0 : 1) : H.dma == 2 ? 2 : H.dma == 1 ? 3 : 4;
+            int per_cu = occ_cache[slot_i].load();
+            if (per_cu < 1) {
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)
+                    per_cu = 2;
+                occ_cache[slot_i].store(per_cu);
+            }
+            const long cap = (long)(H.ncu > 0 ? H.ncu : 256) * per_cu * (wg ? 1 : 4);
             for (long m = 1;; ++m) {
                 const long nseg = m * cap / nrun;
                 if (nseg < 1) continue;
@@ -2027,24 +2045,9 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const long units = (long)nrun * ((A.nyu + R - 1) / R);
         if (H.cfg.limiter == ZK_LIMITER_TVB || H.cfg.limiter == ZK_LIMITER_LEARNED) A.zsok = H.zsok;
         ProfScope ps(H, "dg_fused");
-        if (wg && H.dma == 3 && H.mf4)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3, true>), dim3((unsigned)units), dim3(256), 0, H.stream,
-                               A, R, nrun);
-        else if (wg && H.dma == 3)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 3, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
-                               A, R, nrun);
-        else if (wg && H.dma == 2)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 2, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
-                               A, R, nrun);
-        else if (wg && H.dma == 1)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 1, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
-                               A, R, nrun);
-        else if (wg)
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, true, 0, false>), dim3((unsigned)units), dim3(256), 0, H.stream,
-                               A, R, nrun);
-        else
-            hipLaunchKernelGGL((dg_fused_kernel<P, 2, false, 0, false>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0,
-                               H.stream, A, R, nrun);
+        const dim3 grid((unsigned)(wg ? units : (units + 3) / 4));
+        void *args[] = {&A, &R, (void *)&nrun};
+        (void)hipLaunchKernel(kern, grid, dim3(256), args, 0, H.stream);
     } else if (ncell > 0) {
         {
             ProfScope ps(H, "dg_face_x");
