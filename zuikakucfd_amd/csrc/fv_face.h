@@ -138,8 +138,8 @@ __device__ __forceinline__ void hllc_fv_f32(const double L[4], const double R[4]
 {
     constexpr double eps = 1e-12;
     const double rhoLd = smax(L[0], eps), rhoRd = smax(R[0], eps);
-    const double pLd = smax((gamma - 1.0) * (L[3] - 0.5 * (L[1] * L[1] + L[2] * L[2]) / rhoLd), eps);
-    const double pRd = smax((gamma - 1.0) * (R[3] - 0.5 * (R[1] * R[1] + R[2] * R[2]) / rhoRd), eps);
+    const double pLd = smax((gamma - 1.0) * (L[3] - div_nz(0.5 * (L[1] * L[1] + L[2] * L[2]), rhoLd)), eps);
+    const double pRd = smax((gamma - 1.0) * (R[3] - div_nz(0.5 * (R[1] * R[1] + R[2] * R[2]), rhoRd)), eps);
     const float g = (float)gamma, gm1 = (float)(gamma - 1.0);
     const float rhoL = (float)rhoLd, rhoR = (float)rhoRd, pL = (float)pLd, pR = (float)pRd;
     const float irL = frcp(rhoL), irR = frcp(rhoR);

@@ -48,14 +48,14 @@ __device__ __forceinline__ bool fv_bad_state(const double U[4], double gamma)
 {
     const double rho = U[0];
     const double q = 2.0 * rho * U[3] - (U[1] * U[1] + U[2] * U[2]);
-    return !(rho > 1e-12 && q > 2.0 * rho * 1e-12 / (gamma - 1.0));
+    return !(rho > 1e-12 && q > div_nz(2.0 * rho * 1e-12, gamma - 1.0));
 }
 
 // Face positivity check in FP64 (an option with no reference counterpart).
 __device__ __forceinline__ bool face_positive(const double Q[4], double gamma)
 {
     if (!(Q[0] >= 1e-12)) return false;
-    const double p = (gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+    const double p = (gamma - 1.0) * (Q[3] - div_nz(0.5 * (Q[1] * Q[1] + Q[2] * Q[2]), Q[0]));  // Q[0] >= 1e-12
     return p >= 1e-12;
 }
 
