@@ -1706,7 +1706,7 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long mask, int want)
 // candidate), and treats the others itself, 16 at a time, with the
 // matrix-core TVB + positivity routine.  Limiting changes no mean, so the
 // cells are independent and the order does not matter.
-template <int P>
+template <int P, int CW>
 __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
                                                           const int32_t *__restrict__ bct,
@@ -1718,12 +1718,12 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
 {
     if (S->done) return;  // uniform over the grid
     const int lane = threadIdx.x & 63, col = lane & 15;
-    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CW;
     if (t0 >= ncell) return;  // whole wave
     const size_t t = t0 + lane;
     bool need = false;
     size_t k = 0;
-    if (t < ncell) {
+    if (lane < CW && t < ncell) {
         k = interior_cell(t, nx, nxu);
         const unsigned z = zsok[k];
         need = ((z & 2) && upd[k]) || !(z & 1);
@@ -2100,7 +2100,11 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // are limited or not certified, and the matrix-core kernel treats only those;
         // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
         if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
-            hipLaunchKernelGGL((dg_tvb_scan_kernel<P>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, H.stream,
+            // cells per wave: at P = 3 at most two passes of the routine per wave
+            // (the passes are long); at P <= 2 the wider scan is cheaper
+            constexpr int CW = P == 3 ? 32 : 64;
+            hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3((unsigned)((ncell + 4 * CW - 1) / (4 * CW))), dim3(256),
+                               0, H.stream,
                                Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
                                H.g.gamma, H.scal);
         } else if (H.no_mfma || P < 3)
