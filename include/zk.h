@@ -168,6 +168,13 @@ zk_status zk_ensemble_get(zk_ensemble *e, int idx, double *U, double *t_out);
 const char *zk_ensemble_last_error(const zk_ensemble *e);
 zk_status zk_ensemble_destroy(zk_ensemble *e);
 
+/* Face reconstruction of the WENO path: ZK_RECON_WENO5 (default, the
+ * reference's weno5_reconstruct) or MUSCL with the minmod, van Leer or
+ * superbee limiter (muscl_reconstruct, src_WENO/fluid.cpp:203-225, which the
+ * reference defines but its solver never calls).  WENO handles only. */
+enum { ZK_RECON_WENO5 = 0, ZK_RECON_MUSCL_MINMOD = 1, ZK_RECON_MUSCL_VANLEER = 2, ZK_RECON_MUSCL_SUPERBEE = 3 };
+zk_status zk_set_reconstruction(zk_handle *h, int kind);
+
 /* Kernel-variant selection for the autotuner (no reference counterpart):
  * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
  * automatic (fused row-marching stage kernel, workgroup form, without the

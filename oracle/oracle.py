@@ -28,6 +28,7 @@ def lib() -> C.CDLL:
         up = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
         i, d = C.c_int, C.c_double
         L.zko_weno5.argtypes = [dp, C.POINTER(d), C.POINTER(d)]
+        L.zko_muscl.argtypes = [dp, i, C.POINTER(d), C.POINTER(d)]
         for n in ("zko_hllc_fv", "zko_hllc_dg", "zko_hll"):
             getattr(L, n).argtypes = [dp, dp, d, dp]
         L.zko_hllc_speeds.argtypes = [i, dp, dp, d, dp]
@@ -87,12 +88,23 @@ def prim_to_cons(gamma, rho, u, v, p) -> np.ndarray:
 
 
 POSITIVITY = 16
+# flux bits selecting MUSCL instead of WENO5 (zk_oracle.h ZKO_RECON_SHIFT)
+MUSCL_MINMOD, MUSCL_VANLEER, MUSCL_SUPERBEE = 1 << 5, 2 << 5, 3 << 5
+
+
+def muscl(s, limiter: int):
+    """(UL, UR) of zko_muscl on the four cells s = (UL2, UL1, UR1, UR2)."""
+    a = np.ascontiguousarray(s, dtype=np.float64)
+    ul, ur = C.c_double(), C.c_double()
+    lib().zko_muscl(a, int(limiter), C.byref(ul), C.byref(ur))
+    return ul.value, ur.value
 
 
 class FV:
     """Finite-volume oracle state: U has shape (4, ny, nx).  flux may be or-ed
     with POSITIVITY to apply the face positivity check and the cell
-    positivity fallback; `fallbacks` counts the bad cells the fallback met."""
+    positivity fallback, and with one of the MUSCL_* bits to reconstruct with
+    MUSCL instead of WENO5; `fallbacks` counts the bad cells the fallback met."""
 
     def __init__(self, da, gamma, U, bctype, flux: int = 0):
         self.U = np.ascontiguousarray(U, dtype=np.float64).copy()

@@ -32,6 +32,42 @@ static inline double stdmin(double a, double b) { return (b < a) ? b : a; }
  * Point kernels
  * ====================================================================== */
 
+/* MUSCL reconstruction at the face between s[1] and s[2] from the four cells
+ * s[0..3] = UL2, UL1, UR1, UR2, with limiter 0 = minmod, 1 = van Leer,
+ * 2 = superbee (muscl_reconstruct, src_WENO/fluid.cpp:203-225; the reference
+ * defines it but its solver never calls it).  UL = UL1 + sigma_L / 2 with
+ * sigma_L = lim(UL1 - UL2, UR1 - UL1); UR = UR1 - sigma_R / 2 with
+ * sigma_R = lim(UR1 - UL1, UR2 - UR1). */
+static double zko_minmod2(double a, double b) { return a * b <= 0.0 ? 0.0 : (fabs(a) < fabs(b) ? a : b); }
+static double zko_vanleer(double a, double b)
+{
+    const double ab = a * b;
+    return ab <= 0.0 ? 0.0 : 2.0 * ab / (a + b);
+}
+static double zko_superbee(double a, double b)
+{
+    const double s1 = zko_minmod2(a, 2.0 * b), s2 = zko_minmod2(2.0 * a, b);
+    return s1 * s2 <= 0.0 ? 0.0 : (fabs(s1) > fabs(s2) ? s1 : s2);
+}
+void zko_muscl(const double s[4], int limiter, double *ULo, double *URo)
+{
+    const double dL = s[1] - s[0], dC = s[2] - s[1], dR = s[3] - s[2];
+    double sl = 0.0, sr = 0.0;
+    switch (limiter) {
+    case 0: sl = zko_minmod2(dL, dC); sr = zko_minmod2(dC, dR); break;
+    case 1: sl = zko_vanleer(dL, dC); sr = zko_vanleer(dC, dR); break;
+    case 2: sl = zko_superbee(dL, dC); sr = zko_superbee(dC, dR); break;
+    default: break;
+    }
+    *ULo = s[1] + 0.5 * sl;
+    *URo = s[2] - 0.5 * sr;
+}
+
+/* Face reconstruction of the finite-volume path from the 6-cell stencil s
+ * (the face between s[2] and s[3]): recon 0 = WENO5, 1..3 = MUSCL with
+ * limiter recon - 1 on s[1..4]. */
+static void zko_reconstruct(int recon, const double s[6], double *UL, double *UR);
+
 /* Jiang-Shu WENO5 at the face between s[2] and s[3]: UL from the left-biased
  * stencil s[0..4], UR from the right-biased stencil s[1..5].
  * Follows src_WENO/fluid.cpp:230-263 term by term. */
@@ -69,6 +105,12 @@ void zko_weno5(const double s[6], double *ULo, double *URo)
     double aR0 = 0.1 / (s0 * s0), aR1 = 0.6 / (s1 * s1), aR2 = 0.3 / (s2 * s2);
     double inv2 = 1.0 / (aR0 + aR1 + aR2);
     *URo = (aR0 * qR0 + aR1 * qR1 + aR2 * qR2) * inv2;
+}
+
+static void zko_reconstruct(int recon, const double s[6], double *UL, double *UR)
+{
+    if (recon == 0) zko_weno5(s, UL, UR);
+    else zko_muscl(s + 1, recon - 1, UL, UR);
 }
 
 /* HLLC flux of the finite-volume solver, normal velocity = component 1.
@@ -357,7 +399,8 @@ void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *b
     const size_t NP = (size_t)ny * nx;
     const double dx = da, dy = da;
     const int pos = (flux & ZKO_POSITIVITY) != 0;
-    flux &= ~ZKO_POSITIVITY;
+    const int recon = (flux >> ZKO_RECON_SHIFT) & 3;
+    flux &= 15;
     flux_fn fl = flux == 1 ? zko_hll : zko_hllc_fv;
     zko_fv_fill_ghosts(ny, nx, bct, U);
     memset(dU, 0, 4 * NP * sizeof(double));
@@ -378,13 +421,13 @@ void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *b
             for (int c = 0; c < 4; ++c) {
                 double s[6];
                 for (int t = 0; t < 6; ++t) s[t] = X[t + 1][c];
-                zko_weno5(s, &QLr[c], &QRr[c]);          /* face j+1/2 */
+                zko_reconstruct(recon, s, &QLr[c], &QRr[c]);  /* face j+1/2 */
                 for (int t = 0; t < 6; ++t) s[t] = X[t][c];
-                zko_weno5(s, &QLl[c], &QRl[c]);          /* face j-1/2 */
+                zko_reconstruct(recon, s, &QLl[c], &QRl[c]);  /* face j-1/2 */
                 for (int t = 0; t < 6; ++t) s[t] = Y[t + 1][c];
-                zko_weno5(s, &QLd[c], &QRd[c]);          /* face i+1/2 */
+                zko_reconstruct(recon, s, &QLd[c], &QRd[c]);  /* face i+1/2 */
                 for (int t = 0; t < 6; ++t) s[t] = Y[t][c];
-                zko_weno5(s, &QLu[c], &QRu[c]);          /* face i-1/2 */
+                zko_reconstruct(recon, s, &QLu[c], &QRu[c]);  /* face i-1/2 */
             }
             if (pos) {
                 positivity_fix(QLr, QRr, X[3], X[4], gamma);

@@ -40,8 +40,12 @@ void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamm
 
 /* ---------------- finite volume (WENO5) ---------------- */
 /* flux: 0 = HLLC (src_WENO/fluid.cpp:268-317), 1 = HLL (src_DG/fluid.cpp:525-580);
- * or-ed with ZKO_POSITIVITY it also applies the face positivity fallback. */
+ * or-ed with ZKO_POSITIVITY it also applies the face positivity fallback, and
+ * with r << ZKO_RECON_SHIFT it reconstructs with MUSCL (r = 1 minmod, 2 van
+ * Leer, 3 superbee) instead of WENO5 (r = 0). */
 #define ZKO_POSITIVITY 16
+#define ZKO_RECON_SHIFT 5
+void zko_muscl(const double s[4], int limiter, double *UL, double *UR);
 void zko_fv_fill_ghosts(int ny, int nx, const int32_t *bct, double *U /*4 planes*/);
 void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *bct,
                        double *U, double dt, int flux, double *dU, const uint8_t *bad);

@@ -42,6 +42,24 @@ def test_zk_weno_fixed_dt(tmp_path):
         np.testing.assert_allclose(txt.reshape(64, 64), o.U[c], rtol=1e-5, atol=1e-12)
 
 
+def test_zk_weno_muscl_option(tmp_path):
+    """--recon muscl-vanleer selects MUSCL (van Leer) for the whole run; an
+    unknown name is rejected before any work."""
+    m = meshio.riemann_2d(48)
+    meshio.write_mesh_folder(m, tmp_path / "mesh")
+    r = run([BIN / "zk_weno", "mesh", "2e-4", "60", "--dump", "--quiet", "--recon", "muscl-vanleer"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = read_dump(tmp_path / "result" / "60" / "state.f64")[:, 0]
+    mm = meshio.read_mesh_folder(tmp_path / "mesh")
+    o = oracle.FV(mm.da, mm.gamma, oracle.prim_to_cons(mm.gamma, mm.rho, mm.u, mm.v, mm.p), mm.bctype,
+                  flux=oracle.MUSCL_VANLEER)
+    o.step(2e-4, 60)
+    for c in range(4):
+        assert rel_l1(got[c], o.U[c]) <= 1e-10
+    r = run([BIN / "zk_weno", "mesh", "2e-4", "5", "--quiet", "--recon", "muscl-koren"], tmp_path)
+    assert r.returncode != 0 and "unknown reconstruction" in r.stderr
+
+
 def test_zk_weno_rank_files(tmp_path):
     """--ranks 4 writes the per-rank files of `mpirun -np 4`; recombined as
     plot.ipynb does they equal the single-rank output."""

@@ -317,6 +317,54 @@ def test_weno_jet_positivity_fp64():
         assert rel_l1(U[c], o.U[c]) <= TOL, c
 
 
+MUSCL_CASES = [(zk.RECON_MUSCL_MINMOD, oracle.MUSCL_MINMOD), (zk.RECON_MUSCL_VANLEER, oracle.MUSCL_VANLEER),
+               (zk.RECON_MUSCL_SUPERBEE, oracle.MUSCL_SUPERBEE)]
+
+
+@pytest.mark.parametrize("kind,obits", MUSCL_CASES)
+def test_weno_muscl_riemann(kind, obits):
+    """MUSCL reconstruction (src_WENO/fluid.cpp:203-225) in place of WENO5 on
+    the four-quadrant problem, 128^2, 100 steps; then a graph replay of the
+    same handle (the choice travels in the captured arguments)."""
+    say(f"weno muscl kind={kind} riemann 128 x100")
+    m = meshio.riemann_2d(128)
+    s = zk.Solver(method=zk.METHOD_WENO)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    s.set_reconstruction(kind)
+    o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype, flux=obits)
+    s.step(2e-4, 100)
+    o.step(2e-4, 100)
+    U = s.get_conserved()
+    for c in range(4):
+        assert rel_l1(U[c], o.U[c]) <= TOL, c
+    # switching back to WENO5 on the same handle gives the WENO5 oracle again
+    s.set_reconstruction(zk.RECON_WENO5)
+    o.flux = 0
+    s.step(2e-4, 20)
+    o.step(2e-4, 20)
+    U = s.get_conserved()
+    for c in range(4):
+        assert rel_l1(U[c], o.U[c]) <= TOL, c
+
+
+def test_weno_muscl_jet_positivity():
+    """Superbee MUSCL with the face positivity check and the cell fallback on
+    the jet, against the oracle with the same options."""
+    say("weno muscl superbee jet positivity 160x80 x120")
+    m = meshio.jet(160, 80)
+    s = zk.Solver(method=zk.METHOD_WENO, positivity=True)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    s.set_reconstruction(zk.RECON_MUSCL_SUPERBEE)
+    o = oracle.FV(m.da, m.gamma, oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p), m.bctype,
+                  flux=oracle.POSITIVITY | oracle.MUSCL_SUPERBEE)
+    dt = 0.25 * m.da / o.max_speed()
+    s.step(dt, 120)
+    o.step(dt, 120)
+    U = s.get_conserved()
+    for c in range(4):
+        assert rel_l1(U[c], o.U[c]) <= TOL, c
+
+
 def test_weno_step_cfl_matches_loop():
     """On-device CFL stepping equals the host loop dt = cfl*h/lambda."""
     say("weno step_cfl")

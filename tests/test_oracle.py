@@ -73,6 +73,57 @@ def test_weno5_fifth_order():
     assert r1 > 4.5 and r2 > 4.5, (errs, r1, r2)
 
 
+# ---------------------------------------------------------------- MUSCL
+@pytest.mark.parametrize("lim", [0, 1, 2])
+@pytest.mark.parametrize("coef", [(1.0,), (0.3, -2.0), (-1.5, 0.25)])
+def test_muscl_exact_for_linears(lim, coef):
+    """On a linear profile all three slopes agree (minmod, the harmonic mean
+    and superbee all return the common slope), so both face values equal the
+    line at the face (cells [0,1] .. [3,4], face at x = 2)."""
+    s = [cell_avg_poly(coef, k) for k in range(4)]
+    exact = sum(c * 2.0 ** n for n, c in enumerate(coef))
+    UL, UR = oracle.muscl(s, lim)
+    assert UL == pytest.approx(exact, rel=1e-14, abs=1e-14)
+    assert UR == pytest.approx(exact, rel=1e-14, abs=1e-14)
+
+
+def test_muscl_hand_values():
+    """Differences dL = 1, dC = 2, dR = 1: minmod takes the smaller (1),
+    van Leer the harmonic mean 2*1*2/3 = 4/3, superbee max(minmod(1, 4),
+    minmod(2, 2)) = 2."""
+    s = [0.0, 1.0, 3.0, 4.0]
+    assert oracle.muscl(s, 0) == pytest.approx((1.5, 2.5), abs=1e-15)
+    assert oracle.muscl(s, 1) == pytest.approx((1 + 2 / 3, 3 - 2 / 3), abs=1e-15)
+    assert oracle.muscl(s, 2) == pytest.approx((2.0, 2.0), abs=1e-15)
+    # an extremum on each side zeroes both slopes: first order
+    for lim in range(3):
+        assert oracle.muscl([0.0, 1.0, 0.0, 1.0], lim) == (1.0, 0.0)
+    # out-of-range limiter: zero slopes, as the reference's switch default
+    assert oracle.muscl(s, 7) == (1.0, 3.0)
+
+
+@pytest.mark.parametrize("lim", [0, 1, 2])
+def test_muscl_mirror_and_bounds(lim):
+    """Reversing the stencil swaps the two face values, and each face value
+    lies between its cell's average and the neighbour's across the face
+    (all three limiters are TVD: slope/2 never overshoots the next cell)."""
+    rng = np.random.default_rng(7 + lim)
+    for _ in range(200):
+        s = rng.normal(size=4)
+        UL, UR = oracle.muscl(s, lim)
+        UL2, UR2 = oracle.muscl(s[::-1], lim)
+        assert UR == pytest.approx(UL2, rel=1e-14, abs=1e-15)
+        assert UL == pytest.approx(UR2, rel=1e-14, abs=1e-15)
+        lo, hi = min(s[1], s[2]), max(s[1], s[2])
+        assert lo - 1e-14 <= UL <= hi + 1e-14
+        assert lo - 1e-14 <= UR <= hi + 1e-14
+        # limiter ordering of the left slope magnitude: minmod <= van Leer <= superbee
+    for _ in range(200):
+        s = rng.normal(size=4)
+        sl = [abs(oracle.muscl(s, k)[0] - s[1]) for k in range(3)]
+        assert sl[0] <= sl[1] + 1e-15 and sl[1] <= sl[2] + 1e-15
+
+
 # ---------------------------------------------------------------- fluxes
 STATES = [(1.0, 0.3, -0.2, 1.0), (0.138, 1.206, -1.206, 0.029), (5.0, 30.0, 0.0, 0.4127), (0.5, -0.7, 2.0, 0.3)]
 

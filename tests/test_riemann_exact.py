@@ -148,6 +148,27 @@ def test_fv_sod_plateaus(axis):
     assert np.abs(trans[3:-3, 3:-3]).max() < 1e-13
 
 
+def test_fv_muscl_sod_plateaus():
+    """MUSCL + HLLC on Sod: the plateaus are matched away from the fronts for
+    every limiter, and the mean density error orders as the limiters'
+    compressiveness does (minmod most diffusive, superbee least)."""
+    nc, T = 200, 0.2
+    x, rho, u, p, bct = sod_grid(nc, 1)
+    h = 1.0 / nc
+    U = oracle.prim_to_cons(G, rho, u, u, p)
+    k = int(np.ceil(T / (0.4 * h / np.sqrt(G))))
+    l1s = []
+    for flux in (oracle.MUSCL_MINMOD, oracle.MUSCL_VANLEER, oracle.MUSCL_SUPERBEE):
+        o = oracle.FV(h, G, U, bct, flux=flux)
+        o.step(T / k, k)
+        plat, l1 = plateau_errors(x, o.U[0][5], o.U[1][5] / o.U[0][5], T)
+        assert plat < 5e-3, (flux, plat)
+        assert np.abs(o.U[2][3:-3, 3:-3]).max() < 1e-13
+        l1s.append(l1)
+    assert l1s[0] > l1s[1] > l1s[2], l1s
+    assert l1s[0] < 6e-3, l1s
+
+
 @pytest.mark.parametrize("P", [1, 2])
 @pytest.mark.parametrize("axis", [1, 0])
 def test_dg_tvb_sod_plateaus(P, axis):

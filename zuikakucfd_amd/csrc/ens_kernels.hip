@@ -71,7 +71,7 @@ __device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I, 
     A.ny = I.ny; A.nx = I.nx; A.np = I.np;
     A.da = I.da; A.gamma = I.gamma;
     A.ca = E.ca; A.cb = E.cb;
-    A.stage = E.stage; A.flux = E.flux; A.positivity = E.positivity;
+    A.stage = E.stage; A.flux = E.flux; A.positivity = E.positivity; A.recon = 0;
     A.tiles_x = I.tiles_x;
     A.ntiles = 0; A.ring = nullptr; A.nring = 0;  // the ensemble has its own ring launch
     A.bad = E.bad ? E.bad + I.cell_off : nullptr;

@@ -100,6 +100,32 @@ __device__ __forceinline__ void hllc_fv_fast_f64(const double L[4], const double
     F[3] = (U3 + p) * u + S * (c * (E + (Sstar - u) * (Sstar + p / m)) - U3);
 }
 
+// MUSCL at the face between UL1 and UR1 from UL2, UL1, UR1, UR2; limiter 0 =
+// minmod, 1 = van Leer, 2 = superbee (muscl_reconstruct, src_WENO/fluid.cpp:
+// 203-225, which the reference defines but its solver never calls; here a
+// selectable reconstruction, FP64 in both precisions).
+__device__ __forceinline__ double minmod2_f64(double a, double b)
+{
+    return a * b <= 0.0 ? 0.0 : (fabs(a) < fabs(b) ? a : b);
+}
+__device__ __forceinline__ double muscl_slope(double a, double b, int limiter)
+{
+    if (limiter == 0) return minmod2_f64(a, b);
+    if (limiter == 1) {
+        const double ab = a * b;
+        return ab <= 0.0 ? 0.0 : 2.0 * ab / (a + b);
+    }
+    const double s1 = minmod2_f64(a, 2.0 * b), s2 = minmod2_f64(2.0 * a, b);
+    return s1 * s2 <= 0.0 ? 0.0 : (fabs(s1) > fabs(s2) ? s1 : s2);
+}
+__device__ __forceinline__ void muscl_f64(double UL2, double UL1, double UR1, double UR2, int limiter, double &UL,
+                                          double &UR)
+{
+    const double dL = UL1 - UL2, dC = UR1 - UL1, dR = UR2 - UR1;
+    UL = UL1 + 0.5 * muscl_slope(dL, dC, limiter);
+    UR = UR1 - 0.5 * muscl_slope(dC, dR, limiter);
+}
+
 // ---------------------------------------------------------------- FP32 bulk
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }

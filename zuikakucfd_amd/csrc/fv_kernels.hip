@@ -94,6 +94,7 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     A.cb = stage == 1 ? 0.25 : (2.0 / 3.0);
     A.flux = H.cfg.flux;
     A.positivity = H.cfg.positivity;
+    A.recon = H.recon;
     const int ux = H.g.nx - 6, uy = H.g.ny - 6;
     A.ring = H.ring;
     A.nring = H.nring;

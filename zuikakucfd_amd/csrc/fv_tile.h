@@ -31,6 +31,7 @@ struct FvArgs {
     int stage;
     int flux;
     int positivity;
+    int recon;           // face reconstruction: 0 WENO5, 1..3 MUSCL (minmod, van Leer, superbee)
     int tiles_x;
     int ntiles;          // blocks past ntiles combine the ghost ring instead
     const int32_t *ring; // cells outside the update range
@@ -74,7 +75,9 @@ __device__ __forceinline__ void fv_face_flux(const FvArgs &A, Get get, double F[
         double w[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) w[k] = get(v, k);
-        if (F32) {
+        if (A.recon != 0) {  // uniform over the grid
+            muscl_f64(w[1], w[2], w[3], w[4], A.recon - 1, QL[v], QR[v]);
+        } else if (F32) {
             const double rf = w[2];
             float dl, drr;
             weno5_pair_f32((float)(w[0] - rf), (float)(w[1] - rf), 0.0f, (float)(w[3] - rf), (float)(w[4] - rf),

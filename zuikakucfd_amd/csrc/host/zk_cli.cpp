@@ -14,6 +14,9 @@
 //   --cfl <nu>               (WENO) auto/time modes use dt = nu*h/lambda instead of the
 //                            reference's dt = lambda*h (src_WENO/solver.cpp:150,169)
 //   --positivity             (WENO) face positivity check with first-order fallback
+//   --recon weno5|muscl-minmod|muscl-vanleer|muscl-superbee
+//                            (WENO) face reconstruction, default weno5 (the reference's
+//                            compiled choice; MUSCL is its muscl_reconstruct, fluid.cpp:203)
 //   --weights <file>         (DG P=2, learned limiter) network parameters, raw little-endian
 //                            float32 (zk_mlp_param_count values); default
 //                            <executable dir>/../data/troubled_cell_p2.f32
@@ -145,6 +148,7 @@ struct Opts {
     bool dump = false;
     bool quiet = false;
     bool positivity = false;
+    int recon = ZK_RECON_WENO5;
     std::string weights;
     int ranks = 1;  // per-rank output files as a `mpirun -np ranks` run would write
     float band_lo = 0.2f, band_hi = 0.8f, audit = 0.02f;
@@ -176,7 +180,16 @@ bool parse_opts(int argc, char **argv, Opts &o)
         else if (k == "--dump") o.dump = true;
         else if (k == "--quiet") o.quiet = true;
         else if (k == "--positivity") o.positivity = true;
-        else {
+        else if (k == "--recon" && val(v)) {
+            if (v == "weno5") o.recon = ZK_RECON_WENO5;
+            else if (v == "muscl-minmod") o.recon = ZK_RECON_MUSCL_MINMOD;
+            else if (v == "muscl-vanleer") o.recon = ZK_RECON_MUSCL_VANLEER;
+            else if (v == "muscl-superbee") o.recon = ZK_RECON_MUSCL_SUPERBEE;
+            else {
+                std::cerr << "unknown reconstruction " << v << "\n";
+                return false;
+            }
+        } else {
             std::cerr << "unknown option " << k << "\n";
             return false;
         }
@@ -300,6 +313,8 @@ int main(int argc, char **argv)
     if (zk_load(h, in.ny, in.nx, in.da, in.gamma, in.rho.data(), in.u.data(), in.v.data(), in.p.data(),
                 in.bct.data()) != ZK_OK)
         return fail(h, "zk_load");
+    if (o.recon != ZK_RECON_WENO5 && zk_set_reconstruction(h, o.recon) != ZK_OK)
+        return fail(h, "zk_set_reconstruction");
     zk_mlp *net = nullptr;
     if (cfg.method == ZK_METHOD_DG && o.limiter == ZK_LIMITER_LEARNED && !attach_network(h, o, &net)) {
         zk_destroy(h);

@@ -636,6 +636,19 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len)
     return ZK_OK;
 }
 
+zk_status zk_set_reconstruction(zk_handle *h, int kind)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    if (H->cfg.method != ZK_METHOD_WENO || kind < ZK_RECON_WENO5 || kind > ZK_RECON_MUSCL_SUPERBEE) {
+        H->err = "zk_set_reconstruction: WENO handle and a ZK_RECON_* value";
+        return ZK_ERR_ARG;
+    }
+    drop_graph(H);  // the captured step holds the old choice in its arguments
+    H->recon = kind;
+    return ZK_OK;
+}
+
 zk_status zk_set_tuning(zk_handle *h, int fv_tile, int dg_variant, int use_graph)
 {
     Handle *H = reinterpret_cast<Handle *>(h);

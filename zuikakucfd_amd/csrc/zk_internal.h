@@ -45,6 +45,7 @@ struct Handle {
     bool loaded = false;
     bool no_mfma = false;     // force the VALU cell kernel (ZK_NO_MFMA=1), for comparison
     int tune = 0;             // kernel variant selector (ZK_TUNE), used by the autotuner
+    int recon = 0;            // WENO face reconstruction: 0 WENO5, 1..3 MUSCL (zk_set_reconstruction)
     int dma = 3;              // workgroup fused DG kernel: 0 tables from constant memory, 1 tables in LDS,
                               // 2 tables in LDS and rows by global-to-LDS loads, 3 the same with two
                               // row buffers (ZK_DMA overrides)

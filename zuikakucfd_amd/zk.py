@@ -21,6 +21,7 @@ METHOD_WENO, METHOD_DG = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
 FLUX_HLLC, FLUX_HLL = 0, 1
 LIMITER_NONE, LIMITER_PERSSON_ZS, LIMITER_TVB, LIMITER_LEARNED = 0, 1, 2, 3
+RECON_WENO5, RECON_MUSCL_MINMOD, RECON_MUSCL_VANLEER, RECON_MUSCL_SUPERBEE = 0, 1, 2, 3
 
 ZK_OK = 0
 _ERR_NAMES = {1: "ZK_ERR_ARG", 2: "ZK_ERR_HIP", 3: "ZK_ERR_NUMERIC", 4: "ZK_ERR_CFL"}
@@ -29,6 +30,7 @@ EXPORTED = [
     "zk_create", "zk_destroy", "zk_last_error", "zk_load", "zk_set_state", "zk_get_state",
     "zk_get_conserved", "zk_max_speed", "zk_step", "zk_step_cfl", "zk_sync", "zk_step_timed",
     "zk_get_flags", "zk_updated_cells", "zk_profile", "zk_profile_report", "zk_set_tuning",
+    "zk_set_reconstruction",
     "zk_ensemble_create", "zk_ensemble_add", "zk_ensemble_finalize", "zk_ensemble_set_tuning",
     "zk_ensemble_step", "zk_ensemble_sync", "zk_ensemble_size", "zk_ensemble_get",
     "zk_ensemble_last_error", "zk_ensemble_destroy",
@@ -89,6 +91,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_profile.argtypes = [vp, C.c_int]
     lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    lib.zk_set_reconstruction.argtypes = [vp, C.c_int]
     lib.zk_ensemble_create.argtypes = [C.POINTER(ZkConfig), C.POINTER(vp)]
     lib.zk_ensemble_add.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, ip, C.c_double]
     lib.zk_ensemble_finalize.argtypes = [vp]
@@ -237,6 +240,10 @@ class Solver:
     def set_tuning(self, fv_tile: int = 0, dg_variant: int = 0, use_graph: bool = True) -> None:
         self._check(self.lib.zk_set_tuning(self.h, int(fv_tile), int(dg_variant), 1 if use_graph else 0),
                     "zk_set_tuning")
+
+    def set_reconstruction(self, kind: int) -> None:
+        """WENO face reconstruction: RECON_WENO5 (default) or RECON_MUSCL_*."""
+        self._check(self.lib.zk_set_reconstruction(self.h, int(kind)), "zk_set_reconstruction")
 
     def attach_mlp(self, mlp: "Mlp", lo: float = 0.2, hi: float = 0.8, audit: float = 0.02) -> None:
         self._mlp = mlp  # keep the network alive while the handle uses it
