@@ -109,3 +109,28 @@ def test_rank_files_round_trip(tmp_path):
     assert sorted(p.name for p in (tmp_path / "r4").iterdir())[:4] == ["U0_0.dat", "U0_1.dat", "U0_2.dat", "U0_3.dat"]
     np.testing.assert_array_equal(meshio.combine_rank_files(tmp_path / "r4", 4),
                                   meshio.combine_rank_files(tmp_path / "r1", 1))
+
+
+def test_center_detonation_layout(tmp_path):
+    """gen.ipynb cell 1: two -1 rings, the third ring bctype 0 (fixed,
+    outside the update range), the hot disc centred, da = 2000/n."""
+    m = meshio.center_detonation(64, R0=100.0)
+    assert m.da == 2000.0 / 64 and m.gamma == 1.4
+    b = m.bctype
+    assert (b[:2] == -1).all() and (b[-2:] == -1).all() and (b[:, :2] == -1).all() and (b[:, -2:] == -1).all()
+    assert (b[2:-2, 2:-2] == 0).all()
+    hot = m.p > 1e9
+    # centres within 100 m of (1000, 1000): |j + 0.5 - 32| da, hand count 32 cells
+    jj = (np.arange(64) + 0.5) * m.da - 1000.0
+    assert hot.sum() == int(((jj[:, None] ** 2 + jj[None, :] ** 2) <= 1e4).sum()) == 32
+    assert (m.rho[hot] == 0.1).all() and (m.rho[~hot] == 1.225).all() and (m.p[~hot] == 101325.0).all()
+    assert np.array_equal(hot, hot.T) and np.array_equal(hot, hot[::-1, ::-1])
+    assert not m.u.any() and not m.v.any()
+    M, Vs = meshio.detonation_shock_speed()
+    assert M == pytest.approx(290.8497, rel=1e-6) and Vs == pytest.approx(M * np.sqrt(1.4 * 101325.0 / 1.225))
+    # the generator entry point writes a folder the reader takes back
+    assert meshio.main(["detonation", str(tmp_path / "d"), "--n", "32"]) == 0
+    back = meshio.read_mesh_folder(tmp_path / "d")
+    ref = meshio.center_detonation(32)
+    assert (back.nx, back.ny) == (32, 32) and np.array_equal(back.bctype, ref.bctype)
+    np.testing.assert_allclose(back.p, ref.p, rtol=1e-10)

@@ -81,3 +81,24 @@ def test_random_draw_with_vacuum_stays_finite():
     U = oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p)
     o = run(U, m.bctype, m.da, oracle.POSITIVITY, 14, 0.5)
     assert o.fallbacks > 0 and np.isfinite(o.U).all()
+
+
+def test_center_detonation_symmetry():
+    """The detonation (pressure ratio 1e5) breaks the plain scheme; with the
+    positivity option it stays finite and keeps the problem's symmetries:
+    transposition exactly (the x and y sweeps are the same arithmetic on
+    swapped components) and the point reflection to rounding."""
+    m = meshio.center_detonation(96, R0=40.0)
+    U = oracle.prim_to_cons(m.gamma, m.rho, m.u, m.v, m.p)
+    o = oracle.FV(m.da, m.gamma, U, m.bctype)
+    dt = 0.4 * m.da / o.max_speed()
+    o.step(dt, 60)
+    assert not np.isfinite(o.U).all()
+    o = oracle.FV(m.da, m.gamma, U, m.bctype, flux=oracle.POSITIVITY)
+    o.step(dt, 60)
+    assert np.isfinite(o.U).all() and o.fallbacks > 0
+    r, mu, mv, E = o.U
+    assert np.array_equal(r, r.T) and np.array_equal(E, E.T) and np.array_equal(mu, mv.T)
+    assert np.abs(r - r[::-1, ::-1]).max() < 1e-9 * r.max()
+    np.testing.assert_allclose(mu, -mu[::-1, ::-1], atol=1e-9 * np.abs(mu).max())
+    assert r.min() > 0

@@ -253,6 +253,28 @@ void save(zk_handle *h, const Input &in, int nm, int step, bool dump, int ranks)
     }
 }
 
+// End-of-run performance report with the quantities of the reference's
+// printPerfReport (src_WENO/solver.cpp:19-86, src_DG/solver.cpp:86-150):
+// wall time, mean time per step, I/O time and its share, and the per-rank
+// compute table with its imbalance factor.  One process drives the whole grid
+// here, so the table has one row and the imbalance is 1.  The mean per step
+// is wall/steps for both methods (the WENO report divides by the rank count).
+void perf_report(double wall_ms, double compute_ms, double io_ms, int steps, long long cells, const std::string &info)
+{
+    const std::string rule(40, '=');
+    std::printf("\n%s\n  performance report\n%s\n", rule.c_str(), rule.c_str());
+    std::printf("  total run time : %12.3f ms  (%.3f s)\n", wall_ms, wall_ms / 1000.0);
+    std::printf("  mean per step  : %12.3f ms/step\n", steps > 0 ? wall_ms / steps : 0.0);
+    std::printf("  max I/O time   : %12.3f ms  (%.1f%% of total)\n", io_ms, wall_ms > 0 ? io_ms / wall_ms * 100.0 : 0.0);
+    std::printf("\n  -- load balance --\n  %6s %14s %10s %s\n  %s\n", "rank", "compute (ms)", "cells", " share",
+                std::string(38, '-').c_str());
+    std::printf("  %6d %14.3f %10lld  [||||||||||] 100.0%%\n", 0, compute_ms, cells);
+    std::printf("  %s\n  max %.3f ms  min %.3f ms  mean %.3f ms\n", std::string(38, '-').c_str(), compute_ms, compute_ms,
+                compute_ms);
+    std::printf("  imbalance      : 1.000\n  efficiency     : 100.0%%\n  run            : %s\n%s\n", info.c_str(),
+                rule.c_str());
+}
+
 using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t0) { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 
@@ -437,7 +459,9 @@ int main(int argc, char **argv)
     int64_t nupd = 0;
     zk_updated_cells(h, &nupd);
     const double wall_ms = ms_since(wall0);
-    std::printf("\nwall %.3f ms  compute %.3f ms  io %.3f ms  steps %d  t %.6f\n", wall_ms, compute_ms, io_ms, step, t);
+    perf_report(wall_ms, compute_ms, io_ms, step, (long long)in.ny * in.nx,
+                (time_mode ? "t_end=" + std::to_string(t_end) : "steps=" + std::to_string(timesteps)) +
+                    "  t=" + std::to_string(t));
     if (compute_ms > 0)
         std::printf("cell-stage updates/s: %.4e (%lld updated cells x 3 stages x %d steps)\n",
                     3.0 * (double)nupd * step / (compute_ms / 1e3), (long long)nupd, step);

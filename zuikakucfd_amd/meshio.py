@@ -238,6 +238,41 @@ def jet(nx: int = 2048, ny: int = 1024, da: float | None = None, gamma: float = 
     return MeshData(nx, ny, da, gamma, rho, u, v, p, bct)
 
 
+def center_detonation(n: int = 1024, gamma: float = 1.4, domain: float = 2000.0, R0: float = 15.0,
+                      rho_in: float = 0.1, p_in: float = 1e10, rho_out: float = 1.225,
+                      p_out: float = 101325.0) -> MeshData:
+    """Centre detonation in SI units (gen.ipynb cell 1, with the sizes its
+    main block uses): a gas at rest, (rho_out, p_out) outside and
+    (rho_in, p_in) inside the disc of radius R0 about the domain's centre
+    (cells whose centre has r <= R0), on an n x n grid with da = domain/n.
+    That notebook marks only the outer TWO rings -1; the third ring keeps
+    bctype 0 outside the update range [3, n-3), so those cells are fixed
+    at the ambient state, as the reference treats them."""
+    da = domain / n
+    c = (np.arange(n) + 0.5) * da
+    XC, YC = np.meshgrid(c, domain - c)
+    inside = (XC - domain / 2) ** 2 + (YC - domain / 2) ** 2 <= R0 ** 2
+    rho = np.where(inside, rho_in, rho_out)
+    p = np.where(inside, p_in, p_out)
+    z = np.zeros((n, n))
+    b = np.zeros((n, n), dtype=np.int32)
+    b[:2, :] = -1
+    b[-2:, :] = -1
+    b[:, :2] = -1
+    b[:, -2:] = -1
+    return MeshData(n, n, da, gamma, rho, z, z.copy(), p, b)
+
+
+def detonation_shock_speed(gamma: float = 1.4, rho_out: float = 1.225, p_out: float = 101325.0,
+                           p_in: float = 1e10) -> tuple[float, float]:
+    """(Mach, speed) of the initial blast front the notebook prints: Mach
+    sqrt(((g+1) eta + g-1) / (2 g)) with eta = p_in/p_out, speed = Mach *
+    the ambient sound speed."""
+    eta = p_in / p_out
+    M = np.sqrt(((gamma + 1) * eta + (gamma - 1)) / (2 * gamma))
+    return float(M), float(M * np.sqrt(gamma * p_out / rho_out))
+
+
 def smooth_wave(n: int = 512, amp: float = 0.2, gamma: float = 1.4) -> MeshData:
     """rho = 1 + amp sin(2 pi (x + y)), u = v = 1, p = 1 on the periodic unit
     square (solver coordinates: x along j, y along +i), ring of -2 ghosts,
@@ -285,3 +320,33 @@ def smooth_wave_modes(n: int, P: int, amp: float = 0.2, gamma: float = 1.4, t: f
     Q[3] = R
     Q[3, 0] += 1.0 / (gamma - 1.0)
     return Q
+
+
+GENERATORS = {
+    "riemann": lambda a: riemann_2d(a.n, gamma=a.gamma or 1.4),
+    "random": lambda a: random_riemann(a.seed, a.n, gamma=a.gamma or 1.4),
+    "jet": lambda a: jet(2 * a.n, a.n, gamma=a.gamma or 5.0 / 3.0),
+    "smooth": lambda a: smooth_wave(a.n, gamma=a.gamma or 1.4),
+    "detonation": lambda a: center_detonation(a.n, gamma=a.gamma or 1.4),
+}
+
+
+def main(argv=None) -> int:
+    """Write an input folder for the solvers:
+    python -m zuikakucfd_amd.meshio <kind> <folder> [--n N] [--seed S] [--gamma G]."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m zuikakucfd_amd.meshio", description=main.__doc__)
+    ap.add_argument("kind", choices=sorted(GENERATORS))
+    ap.add_argument("folder")
+    ap.add_argument("--n", type=int, default=256, help="cells per side (jet: rows; columns are 2n)")
+    ap.add_argument("--seed", type=int, default=0, help="random: seed of the four-quadrant draw")
+    ap.add_argument("--gamma", type=float, default=None)
+    a = ap.parse_args(argv)
+    m = GENERATORS[a.kind](a)
+    write_mesh_folder(m, a.folder)
+    print(f"{a.kind}: {m.ny} x {m.nx}, da={m.da:.6g}, gamma={m.gamma} -> {os.path.abspath(a.folder)}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
