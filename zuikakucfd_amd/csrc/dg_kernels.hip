@@ -24,6 +24,7 @@
 #include "zk_internal.h"
 #include "zk_device.h"
 #include "mlp_device.h"
+#include "tile_map.h"
 
 namespace zk {
 // learned indicator hooks (mlp_kernels.hip)
@@ -1007,8 +1008,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     bool store_col;
     if (WG) {
         if ((int)blockIdx.x >= nrun * nseg) return;  // whole workgroup
-        run = blockIdx.x % nrun;
-        seg = blockIdx.x / nrun;
+        // neighbouring runs of a segment on one XCD: the cache lines that
+        // straddle two runs' columns are then fetched into one L2, not two
+        const int t = xcd_tile((int)blockIdx.x, nrun * nseg);
+        run = t % nrun;
+        seg = t / nrun;
         const int x = 16 * wv + col;
         j = 2 + run * 62 + x;
         store_col = x >= 1 && x <= 62 && j <= A.nx - 4;
@@ -1296,9 +1300,12 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             }
         }
         if (!inner) break;
-        // U^n of row r, needed at the end of this iteration
+        // U^n of row r, needed at the end of this iteration.  In the first
+        // stage U^n is the row just read (an updated cell's ghost-map entry is
+        // itself, unsigned), so only the cells that are not updated read it.
+        const bool upd = A.upd[k] != 0;
         double qn[4][4];
-        if (store_col && slot)
+        if (store_col && slot && !(A.stage == 0 && upd))
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -1416,7 +1423,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             }
         }
         // combination of row r up to its S-face term
-        const bool upd = A.upd[k] != 0;
         const double cdt = (A.stage == 0) ? dt : A.cb * dt;
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
@@ -1427,7 +1433,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v <= P; ++v) {
                     const double du = upd ? acc[c][v] * fdu[v] : 0.0;
-                    base[c][v] = (A.stage == 0) ? qn[c][v] + dt * du : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
+                    base[c][v] = (A.stage == 0) ? (upd ? Q[c][v] + dt * du : qn[c][v])
+                                                : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
                 }
         }
         kp = k;
