@@ -47,11 +47,15 @@ def test_documented_errors():
         s.set_tuning(3, 0)
     with pytest.raises(zk.ZkError, match="dg_variant"):
         s.set_tuning(0, 5)
+    with pytest.raises(zk.ZkError, match="ZK_RECON"):
+        s.set_reconstruction(4)
     s.close()
     d = zk.Solver(method=zk.METHOD_DG, P=2, limiter=zk.LIMITER_LEARNED)
     d.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
     with pytest.raises(zk.ZkError, match="network"):
         d.step(1e-4, 1)
+    with pytest.raises(zk.ZkError, match="WENO handle"):
+        d.set_reconstruction(zk.RECON_MUSCL_MINMOD)
     d.close()
     t = zk.Solver(method=zk.METHOD_DG, P=2, limiter=zk.LIMITER_TVB)
     t.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)

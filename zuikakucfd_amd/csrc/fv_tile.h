@@ -18,7 +18,7 @@ namespace zk {
 constexpr int FV_THREADS = 256;
 
 struct FvArgs {
-    const double *Un;
+    const double *Un;    // U^n; the first stage (stage 0) is called with Uin == Un
     const double *Uin;
     double *Uout;
     const int32_t *src;
@@ -214,7 +214,9 @@ __device__ __forceinline__ void fv_tile_body(const FvArgs &A, int tile, double (
         for (int v = 0; v < 4; ++v) {
             double dU = 0.0;
             if (upd) dU = -(xs[q][v] + dtdy * (s_f[v][(r + 1) * TX + c] - s_f[v][r * TX + c]));
-            const double un = A.Un[v * np + k];
+            // the first stage runs with Uin == Un, and an updated cell's ghost-map
+            // entry is itself: its U^n is the staged value, not read again
+            const double un = (A.stage == 0 && upd) ? s_u[v][r + 3][c + 3] : A.Un[v * np + k];
             out[v] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (s_u[v][r + 3][c + 3] + dU);
         }
         if (A.bad) {
