@@ -175,6 +175,16 @@ zk_status zk_ensemble_destroy(zk_ensemble *e);
 enum { ZK_RECON_WENO5 = 0, ZK_RECON_MUSCL_MINMOD = 1, ZK_RECON_MUSCL_VANLEER = 2, ZK_RECON_MUSCL_SUPERBEE = 3 };
 zk_status zk_set_reconstruction(zk_handle *h, int kind);
 
+/* Slip walls on the WENO path (no reference counterpart: the reference's
+ * finite-volume fill, src_WENO/fluid.cpp:338-370, leaves bctype-1 cells
+ * untouched, which stays the default).  With enable = 1 a bctype-1 cell
+ * within 3 cells of an updated cell along a row or column takes the updated
+ * cell mirrored across the wall between them (2k-1 cells away when the
+ * nearest updated cell is k away), with the momentum normal to the wall
+ * negated; the column pass comes first and the row pass overrides it, as in
+ * the reference's fills.  Before or after zk_load; WENO handles only. */
+zk_status zk_set_slip_walls(zk_handle *h, int enable);
+
 /* Kernel-variant selection for the autotuner (no reference counterpart):
  * fv_tile 0 = automatic, 1 = 16x15, 2 = 32x15 WENO tiles; dg_variant 0 =
  * automatic (fused row-marching stage kernel, workgroup form, without the

@@ -33,6 +33,7 @@ def lib() -> C.CDLL:
             getattr(L, n).argtypes = [dp, dp, d, dp]
         L.zko_hllc_speeds.argtypes = [i, dp, dp, d, dp]
         L.zko_fv_fill_ghosts.argtypes = [i, i, ip, dp]
+        L.zko_fv_fill_ghosts_slip.argtypes = [i, i, ip, dp]
         L.zko_fv_rhs.argtypes = [i, i, d, d, ip, dp, d, i, dp]
         L.zko_fv_step.argtypes = [i, i, d, d, ip, dp, d, i]
         L.zko_fv_step.restype = i
@@ -90,6 +91,8 @@ def prim_to_cons(gamma, rho, u, v, p) -> np.ndarray:
 POSITIVITY = 16
 # flux bits selecting MUSCL instead of WENO5 (zk_oracle.h ZKO_RECON_SHIFT)
 MUSCL_MINMOD, MUSCL_VANLEER, MUSCL_SUPERBEE = 1 << 5, 2 << 5, 3 << 5
+# flux bit making bctype-1 cells mirror slip walls (zk_oracle.h ZKO_FV_SLIP)
+FV_SLIP = 1 << 7
 
 
 def muscl(s, limiter: int):
@@ -103,8 +106,9 @@ def muscl(s, limiter: int):
 class FV:
     """Finite-volume oracle state: U has shape (4, ny, nx).  flux may be or-ed
     with POSITIVITY to apply the face positivity check and the cell
-    positivity fallback, and with one of the MUSCL_* bits to reconstruct with
-    MUSCL instead of WENO5; `fallbacks` counts the bad cells the fallback met."""
+    positivity fallback, with one of the MUSCL_* bits to reconstruct with
+    MUSCL instead of WENO5, and with FV_SLIP to make bctype-1 cells mirror
+    slip walls; `fallbacks` counts the bad cells the fallback met."""
 
     def __init__(self, da, gamma, U, bctype, flux: int = 0):
         self.U = np.ascontiguousarray(U, dtype=np.float64).copy()
@@ -124,7 +128,8 @@ class FV:
         return dU
 
     def fill(self) -> None:
-        lib().zko_fv_fill_ghosts(self.ny, self.nx, self.bct, self.U)
+        f = lib().zko_fv_fill_ghosts_slip if self.flux & FV_SLIP else lib().zko_fv_fill_ghosts
+        f(self.ny, self.nx, self.bct, self.U)
 
     def max_speed(self) -> float:
         return lib().zko_fv_max_speed(self.ny, self.nx, self.gamma, self.U)

@@ -316,10 +316,45 @@ static inline int per_idx(int i, int n)
     return i;
 }
 
+/* Mirror slip wall of the finite-volume option, one pass (0: along columns,
+ * wall normal y, negate rho*v; 1: along rows, negate rho*u).  A bctype-1 cell
+ * whose nearest updated cell along the pass lies k <= 3 cells away (ties go
+ * to the larger index) takes the cell mirrored across the wall between them,
+ * k-1 cells past that updated cell (2k-1 from the ghost), or the nearest
+ * updated cell itself where the mirrored one is not updated. */
+static void fill_slip_mirror(int ny, int nx, const int32_t *bct, double *U, int pass)
+{
+    const size_t NP = (size_t)ny * nx;
+    const int n = pass == 0 ? ny : nx;
+#define ZKO_UPD(a) (pass == 0 ? ((a) >= 3 && (a) < ny - 3 && j >= 3 && j < nx - 3 && bct[IDX((a), j)] == 0) \
+                              : ((a) >= 3 && (a) < nx - 3 && i >= 3 && i < ny - 3 && bct[IDX(i, (a))] == 0))
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            if (bct[IDX(i, j)] != 1) continue;
+            const int g = pass == 0 ? i : j;
+            int k = 0, s = 0;
+            for (int d = 1; d <= 3 && !k; ++d) {
+                if (g + d < n && ZKO_UPD(g + d)) { k = d; s = 1; }
+                else if (g - d >= 0 && ZKO_UPD(g - d)) { k = d; s = -1; }
+            }
+            if (!k) continue;
+            int m = g + s * (2 * k - 1);
+            if (!(m >= 0 && m < n && ZKO_UPD(m))) m = g + s * k;
+            const size_t src = pass == 0 ? IDX(m, j) : IDX(i, m), dst = IDX(i, j);
+            for (int c = 0; c < 4; ++c) {
+                const double x = U[(size_t)c * NP + src];
+                U[(size_t)c * NP + dst] = (c == (pass == 0 ? 2 : 1)) ? -x : x;
+            }
+        }
+#undef ZKO_UPD
+}
+
 /* Copy ghosts of nplanes planes: the row pass then the column pass of
  * src_WENO/fluid.cpp:338-370 / src_DG/fluid.cpp:581-635 (last writer wins),
- * with slip (bctype 1, DG only) negating the normal momentum planes
- * [plane_c1, plane_c1+nm) or [plane_c2, ...), then the periodic (-2) pass. */
+ * with slip (bctype 1) = 1 (DG) negating the normal momentum planes
+ * [plane_c1, plane_c1+nm) or [plane_c2, ...), or slip = 2 (finite volume,
+ * nm = 1) filling mirror slip walls after each pass; then the periodic (-2)
+ * pass. */
 static void fill_ghosts(int ny, int nx, const int32_t *bct, double *Q, int nm, int slip)
 {
     const size_t NP = (size_t)ny * nx;
@@ -336,7 +371,7 @@ static void fill_ghosts(int ny, int nx, const int32_t *bct, double *Q, int nm, i
                     size_t src = IDX(i, j), dst = IDX(ni, nj);
                     if (bt == -1) {
                         for (int p = 0; p < 4 * nm; ++p) Q[p * NP + dst] = Q[p * NP + src];
-                    } else if (bt == 1 && slip) {
+                    } else if (bt == 1 && slip == 1) {
                         int neg_c = (pass == 0) ? 2 : 1; /* y pass: rho*v, x pass: rho*u */
                         for (int c = 0; c < 4; ++c)
                             for (int m = 0; m < nm; ++m) {
@@ -346,6 +381,7 @@ static void fill_ghosts(int ny, int nx, const int32_t *bct, double *Q, int nm, i
                     }
                 }
             }
+        if (slip == 2) fill_slip_mirror(ny, nx, bct, Q, pass);
     }
     for (int i = 0; i < ny; ++i)
         for (int j = 0; j < nx; ++j) {
@@ -358,6 +394,11 @@ static void fill_ghosts(int ny, int nx, const int32_t *bct, double *Q, int nm, i
 void zko_fv_fill_ghosts(int ny, int nx, const int32_t *bct, double *U)
 {
     fill_ghosts(ny, nx, bct, U, 1, 0);
+}
+
+void zko_fv_fill_ghosts_slip(int ny, int nx, const int32_t *bct, double *U)
+{
+    fill_ghosts(ny, nx, bct, U, 1, 2);
 }
 
 void zko_dg_fill_ghosts(int ny, int nx, int P, const int32_t *bct, double *Q)
@@ -400,11 +441,24 @@ void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *b
     const double dx = da, dy = da;
     const int pos = (flux & ZKO_POSITIVITY) != 0;
     const int recon = (flux >> ZKO_RECON_SHIFT) & 3;
+    const int slip = (flux & ZKO_FV_SLIP) != 0;
     flux &= 15;
     flux_fn fl = flux == 1 ? zko_hll : zko_hllc_fv;
-    zko_fv_fill_ghosts(ny, nx, bct, U);
+    fill_ghosts(ny, nx, bct, U, 1, slip ? 2 : 0);
     memset(dU, 0, 4 * NP * sizeof(double));
     const double *p0 = U, *p1 = U + NP, *p2 = U + 2 * NP, *p3 = U + 3 * NP;
+    /* Slip walls: a wall cell that lies within 3 cells of updated cells along
+     * both its row and its column holds the row mirror (the row pass comes
+     * last), which is the right ghost for the x faces; the y faces read a copy
+     * in which the column pass is applied again. */
+    double *Uy = NULL;
+    const double *q0 = p0, *q1 = p1, *q2 = p2, *q3 = p3;
+    if (slip) {
+        Uy = (double *)malloc(4 * NP * sizeof(double));
+        memcpy(Uy, U, 4 * NP * sizeof(double));
+        fill_slip_mirror(ny, nx, bct, Uy, 0);
+        q0 = Uy; q1 = Uy + NP; q2 = Uy + 2 * NP; q3 = Uy + 3 * NP;
+    }
 
 #pragma omp parallel for schedule(dynamic, 8)
     for (int i = 3; i < ny - 3; ++i) {
@@ -415,7 +469,7 @@ void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *b
             for (int o = -3; o <= 3; ++o) {
                 size_t kx = IDX(i, j + o), ky = IDX(i + o, j);
                 X[o + 3][0] = p0[kx]; X[o + 3][1] = p1[kx]; X[o + 3][2] = p2[kx]; X[o + 3][3] = p3[kx];
-                Y[o + 3][0] = p0[ky]; Y[o + 3][1] = p1[ky]; Y[o + 3][2] = p2[ky]; Y[o + 3][3] = p3[ky];
+                Y[o + 3][0] = q0[ky]; Y[o + 3][1] = q1[ky]; Y[o + 3][2] = q2[ky]; Y[o + 3][3] = q3[ky];
             }
             double QLr[4], QRr[4], QLl[4], QRl[4], QLd[4], QRd[4], QLu[4], QRu[4];
             for (int c = 0; c < 4; ++c) {
@@ -463,6 +517,7 @@ void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *b
             dU[3 * NP + k] = -(dtdx * (Fr[3] - Fl[3]) + dtdy * (Fd[3] - Fu[3]));
         }
     }
+    free(Uy);
 }
 
 void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,

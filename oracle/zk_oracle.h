@@ -45,8 +45,12 @@ void zko_hllc_speeds(int kind, const double L[4], const double R[4], double gamm
  * Leer, 3 superbee) instead of WENO5 (r = 0). */
 #define ZKO_POSITIVITY 16
 #define ZKO_RECON_SHIFT 5
+/* or-ed into flux: bctype-1 cells become mirror slip walls (no reference
+ * counterpart; the reference's finite-volume fill leaves them untouched) */
+#define ZKO_FV_SLIP 128
 void zko_muscl(const double s[4], int limiter, double *UL, double *UR);
 void zko_fv_fill_ghosts(int ny, int nx, const int32_t *bct, double *U /*4 planes*/);
+void zko_fv_fill_ghosts_slip(int ny, int nx, const int32_t *bct, double *U /*4 planes*/);
 void zko_fv_rhs_masked(int ny, int nx, double da, double gamma, const int32_t *bct,
                        double *U, double dt, int flux, double *dU, const uint8_t *bad);
 void zko_fv_rhs(int ny, int nx, double da, double gamma, const int32_t *bct,

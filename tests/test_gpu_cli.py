@@ -60,6 +60,22 @@ def test_zk_weno_muscl_option(tmp_path):
     assert r.returncode != 0 and "unknown reconstruction" in r.stderr
 
 
+def test_zk_weno_slip_option(tmp_path):
+    """--slip makes bctype-1 cells mirror slip walls for the run."""
+    m = meshio.riemann_2d(40, split=(0.6, 0.6))
+    m.bctype = meshio.ring_bctype(40, 40, 1)
+    meshio.write_mesh_folder(m, tmp_path / "mesh")
+    r = run([BIN / "zk_weno", "mesh", "5e-4", "40", "--dump", "--quiet", "--slip"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = read_dump(tmp_path / "result" / "40" / "state.f64")[:, 0]
+    mm = meshio.read_mesh_folder(tmp_path / "mesh")
+    o = oracle.FV(mm.da, mm.gamma, oracle.prim_to_cons(mm.gamma, mm.rho, mm.u, mm.v, mm.p), mm.bctype,
+                  flux=oracle.FV_SLIP)
+    o.step(5e-4, 40)
+    for c in range(4):
+        assert rel_l1(got[c], o.U[c]) <= 1e-10
+
+
 def test_zk_weno_rank_files(tmp_path):
     """--ranks 4 writes the per-rank files of `mpirun -np 4`; recombined as
     plot.ipynb does they equal the single-rank output."""

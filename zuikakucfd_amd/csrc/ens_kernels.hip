@@ -67,6 +67,7 @@ __device__ __forceinline__ FvArgs make_args(const EnsArgs &E, const EnsInst &I, 
     A.Uin = E.Uin + I.off;
     A.Uout = E.Uout + I.off;
     A.src = E.src + I.cell_off;
+    A.srcy = nullptr;
     A.upd = E.upd + I.cell_off;
     A.ny = I.ny; A.nx = I.nx; A.np = I.np;
     A.da = I.da; A.gamma = I.gamma;

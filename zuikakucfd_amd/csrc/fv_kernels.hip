@@ -87,6 +87,7 @@ void fv_stage(Handle &H, const double *Un, const double *Uin, double *Uout, int 
     FvArgs A;
     A.Un = Un; A.Uin = Uin; A.Uout = Uout;
     A.src = H.src; A.upd = H.upd; A.S = H.scal;
+    A.srcy = H.fv_slip ? H.srcy : nullptr;
     A.ny = H.g.ny; A.nx = H.g.nx; A.np = H.g.np;
     A.da = H.g.da; A.gamma = H.g.gamma;
     A.stage = stage;

@@ -22,6 +22,7 @@ struct FvArgs {
     const double *Uin;
     double *Uout;
     const int32_t *src;
+    const int32_t *srcy; // slip walls on: the ghost map of the y faces (column mirrors), else null
     const uint8_t *upd;
     const StepScalars *S;
     int ny, nx;
@@ -196,6 +197,25 @@ __device__ __forceinline__ void fv_tile_body(const FvArgs &A, int tile, double (
         for (int v = 0; v < 4; ++v)
             xs[q][v] = (t < TX * TY) ? dtdx * (s_f[v][r * (TX + 1) + c + 1] - s_f[v][r * (TX + 1) + c]) : 0.0;
     }
+    if (A.srcy) {
+        // slip walls: a wall cell near updated cells along both its row and
+        // its column holds the row mirror; the y faces take the column mirror
+        for (int t = tid; t < SH * SW; t += FV_THREADS) {
+            const int r = t / SW, c = t % SW;
+            const int i = i0 - 3 + r, j = j0 - 3 + c;
+            if (i < ny && j < nx) {
+                const size_t kk = (size_t)i * nx + j;
+                const int32_t e = A.srcy[kk];
+                if (e != A.src[kk]) {
+                    const size_t s = (size_t)(e >> kSrcShift);
+                    double v0 = A.Uin[s], v1 = A.Uin[np + s], v2 = A.Uin[2 * np + s], v3 = A.Uin[3 * np + s];
+                    if (e & 1) v1 = -v1;
+                    if (e & 2) v2 = -v2;
+                    s_u[0][r][c] = v0; s_u[1][r][c] = v1; s_u[2][r][c] = v2; s_u[3][r][c] = v3;
+                }
+            }
+        }
+    }
     __syncthreads();
     fv_sweep<TX, TY, F32, false>(A, s_u, s_f, i0, j0);
     __syncthreads();
@@ -217,7 +237,15 @@ __device__ __forceinline__ void fv_tile_body(const FvArgs &A, int tile, double (
             // the first stage runs with Uin == Un, and an updated cell's ghost-map
             // entry is itself: its U^n is the staged value, not read again
             const double un = (A.stage == 0 && upd) ? s_u[v][r + 3][c + 3] : A.Un[v * np + k];
-            out[v] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (s_u[v][r + 3][c + 3] + dU);
+            double self = s_u[v][r + 3][c + 3];
+            if (A.srcy && !upd) {
+                // a wall cell inside the update range may hold its column
+                // mirror now; its stored mixture uses the ghost map's value
+                const int32_t e = A.src[k];
+                self = A.Uin[v * np + (size_t)(e >> kSrcShift)];
+                if ((v == 1 && (e & 1)) || (v == 2 && (e & 2))) self = -self;
+            }
+            out[v] = (A.stage == 0) ? un + dU : A.ca * un + A.cb * (self + dU);
         }
         if (A.bad) {
             // cell positivity fallback, pass 1: a bad cell is left unwritten
@@ -255,15 +283,23 @@ __device__ __forceinline__ void fv_fallback_cell(const FvArgs &A, int i, int j)
     if (!A.upd[k]) return;
     const bool b0 = A.bad[k], bE = A.bad[k + 1], bW = A.bad[k - 1], bS = A.bad[k + nx], bN = A.bad[k - nx];
     if (!(b0 || bE || bW || bS || bN)) return;
-    auto U = [&](size_t kk, int v) {
-        const int32_t e = A.src[kk];
+    // the y faces read the column-mirror map where slip walls are on
+    const int32_t *mapy = A.srcy ? A.srcy : A.src;
+    auto Um = [&](const int32_t *map, size_t kk, int v) {
+        const int32_t e = map[kk];
         double x = A.Uin[v * np + (size_t)(e >> kSrcShift)];
         if ((v == 1 && (e & 1)) || (v == 2 && (e & 2))) x = -x;
         return x;
     };
+    auto U = [&](size_t kk, int v) { return Um(A.src, kk, v); };
+    auto Uy = [&](size_t kk, int v) { return Um(mapy, kk, v); };
     auto cell = [&](size_t kk, double c[4]) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) c[v] = U(kk, v);
+    };
+    auto celly = [&](size_t kk, double c[4]) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = Uy(kk, v);
     };
     const double dt = A.S->dt, dtdx = dt / A.da, dtdy = dt / A.da;
     // face (a | b) along x at row i, columns jl | jl+1, or along y at column j, rows il | il+1
@@ -275,15 +311,15 @@ __device__ __forceinline__ void fv_fallback_cell(const FvArgs &A, int i, int j)
     };
     auto fo_y = [&](int il, double F[4]) {
         double a[4], b[4];
-        cell((size_t)il * nx + j, a);
-        cell((size_t)(il + 1) * nx + j, b);
+        celly((size_t)il * nx + j, a);
+        celly((size_t)(il + 1) * nx + j, b);
         fv_face_flux_fo<false>(A, a, b, F);
     };
     auto hi_x = [&](int jl, double F[4]) {
         fv_face_flux<F32, true>(A, [&](int v, int t) { return U((size_t)i * nx + (jl - 2 + t), v); }, F);
     };
     auto hi_y = [&](int il, double F[4]) {
-        fv_face_flux<F32, false>(A, [&](int v, int t) { return U((size_t)(il - 2 + t) * nx + j, v); }, F);
+        fv_face_flux<F32, false>(A, [&](int v, int t) { return Uy((size_t)(il - 2 + t) * nx + j, v); }, F);
     };
     if (b0) {
         double FE[4], FW[4], GS[4], GN[4];

@@ -14,6 +14,8 @@
 //   --cfl <nu>               (WENO) auto/time modes use dt = nu*h/lambda instead of the
 //                            reference's dt = lambda*h (src_WENO/solver.cpp:150,169)
 //   --positivity             (WENO) face positivity check with first-order fallback
+//   --slip                   (WENO) bctype-1 cells are mirror slip walls (zk_set_slip_walls);
+//                            without it they are left alone, as the reference's WENO does
 //   --recon weno5|muscl-minmod|muscl-vanleer|muscl-superbee
 //                            (WENO) face reconstruction, default weno5 (the reference's
 //                            compiled choice; MUSCL is its muscl_reconstruct, fluid.cpp:203)
@@ -149,6 +151,7 @@ struct Opts {
     bool quiet = false;
     bool positivity = false;
     int recon = ZK_RECON_WENO5;
+    bool slip = false;
     std::string weights;
     int ranks = 1;  // per-rank output files as a `mpirun -np ranks` run would write
     float band_lo = 0.2f, band_hi = 0.8f, audit = 0.02f;
@@ -180,6 +183,7 @@ bool parse_opts(int argc, char **argv, Opts &o)
         else if (k == "--dump") o.dump = true;
         else if (k == "--quiet") o.quiet = true;
         else if (k == "--positivity") o.positivity = true;
+        else if (k == "--slip") o.slip = true;
         else if (k == "--recon" && val(v)) {
             if (v == "weno5") o.recon = ZK_RECON_WENO5;
             else if (v == "muscl-minmod") o.recon = ZK_RECON_MUSCL_MINMOD;
@@ -332,6 +336,7 @@ int main(int argc, char **argv)
     cfg.positivity = o.positivity ? 1 : 0;
     zk_handle *h = nullptr;
     if (zk_create(&cfg, &h) != ZK_OK) return fail(nullptr, "zk_create");
+    if (o.slip && zk_set_slip_walls(h, 1) != ZK_OK) return fail(h, "zk_set_slip_walls");
     if (zk_load(h, in.ny, in.nx, in.da, in.gamma, in.rho.data(), in.u.data(), in.v.data(), in.p.data(),
                 in.bct.data()) != ZK_OK)
         return fail(h, "zk_load");

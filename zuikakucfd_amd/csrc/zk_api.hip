@@ -91,12 +91,12 @@ void drop_graph(Handle *H)
 void free_all(Handle *H)
 {
     drop_graph(H);
-    void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
+    void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->srcy, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
                     H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand, H->bad, H->any, H->blist, H->zsok, H->tlist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
-    H->src = nullptr; H->upd = nullptr; H->ring = nullptr;
+    H->src = nullptr; H->srcy = nullptr; H->upd = nullptr; H->ring = nullptr;
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
     H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
     H->stats = nullptr; H->score = nullptr; H->cand = nullptr; H->bad = nullptr; H->any = nullptr; H->blist = nullptr; H->zsok = nullptr; H->tlist = nullptr;
@@ -107,7 +107,40 @@ void free_all(Handle *H)
 // reference fill (src_WENO/fluid.cpp:338-370, src_DG/fluid.cpp:581-635) on
 // indices, so the last writer of every ghost is recorded; then periodic (-2)
 // ghosts take their periodic image inside [3, n-3).
-void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, bool slip, std::vector<int32_t> &src)
+// Mirror slip walls of the finite-volume option (no reference counterpart),
+// one pass: pass 0 along columns (wall normal y, negate rho*v), pass 1 along
+// rows (negate rho*u).  A bctype-1 cell whose nearest updated cell along the
+// pass is k <= 3 away (ties to the larger index) takes the cell mirrored
+// across the wall between them, 2k-1 away, or that nearest updated cell where
+// the mirrored one is not updated.
+void slip_mirror_pass(const std::vector<int32_t> &bct, int ny, int nx, int pass, std::vector<int32_t> &src)
+{
+    const int n = pass == 0 ? ny : nx;
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            if (bct[(size_t)i * nx + j] != 1) continue;
+            auto cell = [&](int a) { return pass == 0 ? (size_t)a * nx + j : (size_t)i * nx + a; };
+            auto updated = [&](int a) {
+                if (a < 3 || a >= n - 3) return false;
+                if (pass == 0 ? (j < 3 || j >= nx - 3) : (i < 3 || i >= ny - 3)) return false;
+                return bct[cell(a)] == 0;
+            };
+            const int g = pass == 0 ? i : j;
+            int k = 0, s = 0;
+            for (int d = 1; d <= 3 && !k; ++d) {
+                if (g + d < n && updated(g + d)) { k = d; s = 1; }
+                else if (g - d >= 0 && updated(g - d)) { k = d; s = -1; }
+            }
+            if (!k) continue;
+            int m = g + s * (2 * k - 1);
+            if (!(m >= 0 && m < n && updated(m))) m = g + s * k;
+            src[(size_t)i * nx + j] = (int32_t)((cell(m) << zk::kSrcShift) | (pass == 0 ? 2 : 1));
+        }
+}
+
+// slip: 0 leaves bctype-1 cells alone (the reference's finite-volume fill),
+// 1 copies with the normal momentum negated (its DG fill), 2 mirror slip walls
+void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, int slip, std::vector<int32_t> &src)
 {
     src.resize((size_t)ny * nx);
     for (size_t k = 0; k < src.size(); ++k) src[k] = (int32_t)(k << zk::kSrcShift);
@@ -129,9 +162,13 @@ void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, bool slip, s
                     const size_t dst = (size_t)ni * nx + nj;
                     const int bt = bct[dst];
                     if (bt == -1) src[dst] = (int32_t)(k << zk::kSrcShift);
-                    else if (bt == 1 && slip) src[dst] = (int32_t)((k << zk::kSrcShift) | (pass == 0 ? 2 : 1));
+                    else if (bt == 1 && slip == 1) src[dst] = (int32_t)((k << zk::kSrcShift) | (pass == 0 ? 2 : 1));
                 }
             }
+    if (slip == 2) {
+        slip_mirror_pass(bct, ny, nx, 0, src);
+        slip_mirror_pass(bct, ny, nx, 1, src);
+    }
     auto per = [](int i, int n) { return i < 3 ? i + (n - 6) : (i >= n - 3 ? i - (n - 6) : i); };
     for (int i = 0; i < ny; ++i)
         for (int j = 0; j < nx; ++j) {
@@ -139,6 +176,16 @@ void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, bool slip, s
             if (bct[k] != -2) continue;
             src[k] = (int32_t)(((size_t)per(i, ny) * nx + per(j, nx)) << zk::kSrcShift);
         }
+}
+
+// The y-face map of the slip-wall option: the ghost map with the column
+// pass of the slip walls applied again, so that a wall cell near updated
+// cells along both its row and its column gives the y faces its column mirror.
+void build_srcy_map(const std::vector<int32_t> &bct, int ny, int nx, const std::vector<int32_t> &src,
+                    std::vector<int32_t> &srcy)
+{
+    srcy = src;
+    slip_mirror_pass(bct, ny, nx, 0, srcy);
 }
 
 }  // namespace
@@ -304,7 +351,7 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     }
     H->h_bct.assign(bctype, bctype + np);
     std::vector<int32_t> src;
-    build_src_map(H->h_bct, ny, nx, H->cfg.method == ZK_METHOD_DG, src);
+    build_src_map(H->h_bct, ny, nx, H->cfg.method == ZK_METHOD_DG ? 1 : (H->fv_slip ? 2 : 0), src);
     std::vector<uint8_t> upd(np, 0);
     std::vector<int32_t> ring;
     int64_t nupd = 0;
@@ -325,6 +372,7 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     ZK_HIP(hipMalloc(&H->qa, nst * sizeof(double)));
     ZK_HIP(hipMalloc(&H->qb, nst * sizeof(double)));
     ZK_HIP(hipMalloc(&H->src, np * sizeof(int32_t)));
+    if (H->cfg.method == ZK_METHOD_WENO) ZK_HIP(hipMalloc(&H->srcy, np * sizeof(int32_t)));
     ZK_HIP(hipMalloc(&H->upd, np));
     ZK_HIP(hipMalloc(&H->bct, np * sizeof(int32_t)));
     ZK_HIP(hipMalloc(&H->ring, ring.size() * sizeof(int32_t)));
@@ -364,6 +412,11 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     ZK_HIP(hipMemcpy(H->qa, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
     ZK_HIP(hipMemcpy(H->qb, Q.data(), nst * sizeof(double), hipMemcpyHostToDevice));
     ZK_HIP(hipMemcpy(H->src, src.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (H->srcy) {
+        std::vector<int32_t> srcy;
+        build_srcy_map(H->h_bct, ny, nx, src, srcy);
+        ZK_HIP(hipMemcpy(H->srcy, srcy.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     ZK_HIP(hipMemcpy(H->upd, upd.data(), np, hipMemcpyHostToDevice));
     ZK_HIP(hipMemcpy(H->bct, bctype, np * sizeof(int32_t), hipMemcpyHostToDevice));
     if (!ring.empty()) ZK_HIP(hipMemcpy(H->ring, ring.data(), ring.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -633,6 +686,28 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len)
         return ZK_ERR_ARG;
     }
     std::memcpy(buf, js.c_str(), js.size() + 1);
+    return ZK_OK;
+}
+
+zk_status zk_set_slip_walls(zk_handle *h, int enable)
+{
+    Handle *H = reinterpret_cast<Handle *>(h);
+    if (!H) return ZK_ERR_ARG;
+    if (H->cfg.method != ZK_METHOD_WENO || (enable != 0 && enable != 1)) {
+        H->err = "zk_set_slip_walls: WENO handle and enable 0 or 1";
+        return ZK_ERR_ARG;
+    }
+    H->fv_slip = enable != 0;
+    if (!H->loaded) return ZK_OK;  // zk_load builds the ghost map with it
+    // rebuild the ghost map of the loaded grid in place
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    drop_graph(H);
+    std::vector<int32_t> src, srcy;
+    build_src_map(H->h_bct, H->g.ny, H->g.nx, H->fv_slip ? 2 : 0, src);
+    build_srcy_map(H->h_bct, H->g.ny, H->g.nx, src, srcy);
+    ZK_HIP(hipMemcpy(H->src, src.data(), src.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(H->srcy, srcy.data(), srcy.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     return ZK_OK;
 }
 

@@ -46,6 +46,7 @@ struct Handle {
     bool no_mfma = false;     // force the VALU cell kernel (ZK_NO_MFMA=1), for comparison
     int tune = 0;             // kernel variant selector (ZK_TUNE), used by the autotuner
     int recon = 0;            // WENO face reconstruction: 0 WENO5, 1..3 MUSCL (zk_set_reconstruction)
+    bool fv_slip = false;     // WENO: bctype-1 cells are mirror slip walls (zk_set_slip_walls)
     int dma = 3;              // workgroup fused DG kernel: 0 tables from constant memory, 1 tables in LDS,
                               // 2 tables in LDS and rows by global-to-LDS loads, 3 the same with two
                               // row buffers (ZK_DMA overrides)
@@ -61,6 +62,7 @@ struct Handle {
     double *qa = nullptr;     // stage buffer A
     double *qb = nullptr;     // stage buffer B
     int32_t *src = nullptr;   // ghost-fill map (ny*nx)
+    int32_t *srcy = nullptr;  // WENO: ghost map of the y faces (column mirrors of slip walls)
     uint8_t *upd = nullptr;   // 1 = updated cell
     int32_t *ring = nullptr;  // indices of cells outside [3,ny-3)x[3,nx-3)
     int nring = 0;

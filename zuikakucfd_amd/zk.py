@@ -31,6 +31,7 @@ EXPORTED = [
     "zk_get_conserved", "zk_max_speed", "zk_step", "zk_step_cfl", "zk_sync", "zk_step_timed",
     "zk_get_flags", "zk_updated_cells", "zk_profile", "zk_profile_report", "zk_set_tuning",
     "zk_set_reconstruction",
+    "zk_set_slip_walls",
     "zk_ensemble_create", "zk_ensemble_add", "zk_ensemble_finalize", "zk_ensemble_set_tuning",
     "zk_ensemble_step", "zk_ensemble_sync", "zk_ensemble_size", "zk_ensemble_get",
     "zk_ensemble_last_error", "zk_ensemble_destroy",
@@ -92,6 +93,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.zk_set_reconstruction.argtypes = [vp, C.c_int]
+    lib.zk_set_slip_walls.argtypes = [vp, C.c_int]
     lib.zk_ensemble_create.argtypes = [C.POINTER(ZkConfig), C.POINTER(vp)]
     lib.zk_ensemble_add.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, ip, C.c_double]
     lib.zk_ensemble_finalize.argtypes = [vp]
@@ -244,6 +246,11 @@ class Solver:
     def set_reconstruction(self, kind: int) -> None:
         """WENO face reconstruction: RECON_WENO5 (default) or RECON_MUSCL_*."""
         self._check(self.lib.zk_set_reconstruction(self.h, int(kind)), "zk_set_reconstruction")
+
+    def set_slip_walls(self, enable: bool) -> None:
+        """WENO: bctype-1 cells become mirror slip walls (off by default, as in
+        the reference)."""
+        self._check(self.lib.zk_set_slip_walls(self.h, 1 if enable else 0), "zk_set_slip_walls")
 
     def attach_mlp(self, mlp: "Mlp", lo: float = 0.2, hi: float = 0.8, audit: float = 0.02) -> None:
         self._mlp = mlp  # keep the network alive while the handle uses it
