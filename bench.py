@@ -317,7 +317,7 @@ def main() -> int:
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: this library has no multi-GPU path "
               "(the slab decomposition is design-only); use --gpus 1.", file=sys.stderr)
         return 2
 

@@ -58,7 +58,7 @@ typedef struct zk_config {
     int precision;   /* ZK_PREC_* */
     int flux;        /* ZK_FLUX_* */
     int limiter;     /* ZK_LIMITER_*; DG only */
-    double tvb_M;    /* TVB constant M (north_star: 50) */
+    double tvb_M;    /* TVB constant M (50 in the benchmark configurations) */
     int device;      /* HIP device ordinal */
     int positivity;  /* WENO: 1 = positivity option (FP64), no reference counterpart;
                         0 reproduces the reference scheme.  Face check: a face whose

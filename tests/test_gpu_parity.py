@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through the C ABI) against the CPU oracle.
 
-Bar (north_star): the plain FP64 path matches the oracle within 1e-10
+Bar: the plain FP64 path matches the oracle within 1e-10
 relative L1 per conserved variable after 100 steps on identical inputs, and
 the TVB limiter flags are identical on the smooth test.  Sizes are chosen so
 the oracle finishes in seconds.
