@@ -1217,9 +1217,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             if constexpr (F4) {
                 // W, E: the modes at eta_g (contracted over py), then L_v(-+1) over v;
                 // N, S: L_v(xi_g) L_py(-+1) accumulated over v on the matrix core
-                double w = 0.0, e = 0.0, n = 0.0, sv = 0.0;
+                const double t0 = mf4(M4.aL[ln], Q[c][0]);
+                double w = t0, e = t0;
+                double n = mf4(M4.aNv[0][ln], Q[c][0]), sv = mf4(M4.aSv[0][ln], Q[c][0]);
 #pragma unroll
-                for (int v = 0; v <= P; ++v) {
+                for (int v = 1; v <= P; ++v) {
                     const double t = mf4(M4.aL[ln], Q[c][v]);
                     w += (v & 1) ? -t : t;
                     e += t;
@@ -1378,9 +1380,12 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 for (int v = 0; v < 4; ++v) t[v] = v <= P ? mf4(M4.aL[ln], Q[c][v]) : 0.0;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    double u = 0.0;
+                    double u = t[0];  // L_0 = 1
 #pragma unroll
-                    for (int v = 0; v <= P; ++v) u += (a < NQ ? leg_c(v, gl_x(NQ, a)) : 0.0) * t[v];
+                    for (int v = 1; v <= P; ++v) {
+                        const double k = a < NQ ? leg_c(v, gl_x(NQ, a)) : 0.0;
+                        if (k != 0.0) u += k * t[v];
+                    }
                     U[c][a] = u;
                 }
             }
@@ -1396,7 +1401,15 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     const double zf = mf4(M4.aWL[ln], F[c]), zg = mf4(M4.aWD[ln], G[c]);
 #pragma unroll
                     for (int v = 0; v <= P; ++v)
-                        acc[c][v] += (gl_w(NQ, a) * legd_c(v, gl_x(NQ, a))) * zf + (gl_w(NQ, a) * leg_c(v, gl_x(NQ, a))) * zg;
+                    {
+                        // the coefficients are constants after unrolling; the zero
+                        // ones (L'_0, and L'_v / L_v at xi = 0 for even / odd v)
+                        // contribute nothing and are left out
+                        const double kf = gl_w(NQ, a) * legd_c(v, gl_x(NQ, a));
+                        const double kg = gl_w(NQ, a) * leg_c(v, gl_x(NQ, a));
+                        if (kf != 0.0) acc[c][v] += kf * zf;
+                        if (kg != 0.0) acc[c][v] += kg * zg;
+                    }
                 }
             }
         } else {
