@@ -1440,15 +1440,36 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
         if (DB && P != 2) reread(Q);
-        if (store_col && slot) {
+        // ca qn + cb (Q + dt du) with du = fdu acc, as ca qn + cb Q + fs acc
+        // (fs = cb dt fdu, 0 for cells not updated); the first stage is
+        // Q + fs acc for updated cells and qn (not read otherwise) for the rest.
+        // At P = 3 the two stage branches cost registers (spills): one
+        // expression there.
+        if constexpr (P == 3) {
+            if (store_col && slot) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v <= P; ++v) {
-                    const double du = upd ? acc[c][v] * fdu[v] : 0.0;
-                    base[c][v] = (A.stage == 0) ? (upd ? Q[c][v] + dt * du : qn[c][v])
-                                                : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
-                }
+                    for (int v = 0; v <= P; ++v) {
+                        const double du = upd ? acc[c][v] * fdu[v] : 0.0;
+                        base[c][v] = (A.stage == 0) ? (upd ? Q[c][v] + dt * du : qn[c][v])
+                                                    : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
+                    }
+            }
+        } else if (store_col && slot) {
+            if (A.stage == 0) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int v = 0; v <= P; ++v) base[c][v] = upd ? fs[v] * acc[c][v] + Q[c][v] : qn[c][v];
+            } else {
+                const double ca = A.ca, cb = A.cb;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int v = 0; v <= P; ++v)
+                        base[c][v] = fs[v] * acc[c][v] + (cb * Q[c][v] + ca * qn[c][v]);
+            }
         }
         kp = k;
     }
