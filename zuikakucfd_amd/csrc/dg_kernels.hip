@@ -196,8 +196,10 @@ __device__ __forceinline__ void euler_fg(const double U[4], double gam, double F
     const double rho = smax(U[0], eps), ir = rcp_nz(rho);
     const double u = U[1] * ir, v = U[2] * ir;
     const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
-    F[0] = rho * u; F[1] = rho * u * u + p; F[2] = rho * u * v; F[3] = (U[3] + p) * u;
-    G[0] = rho * v; G[1] = rho * u * v; G[2] = rho * v * v + p; G[3] = (U[3] + p) * v;
+    // rho u and rho v are the momenta (to rounding, also where rho is floored)
+    const double ruv = U[1] * v, h = U[3] + p;
+    F[0] = U[1]; F[1] = U[1] * u + p; F[2] = ruv; F[3] = h * u;
+    G[0] = U[2]; G[1] = ruv; G[2] = U[2] * v + p; G[3] = h * v;
 }
 
 template <int P>
@@ -1295,8 +1297,10 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 const double tol = 1e-12 * (fabs(mean0[0]) + sab[0] + fabs(mean0[3]) + sab[3] + mx + my);
                 bool ok = rlo > 2e-5 + tol;
                 if (ok) {
-                    const double plo = (gam - 1.0) * (elo - 0.5 * (mx * mx + my * my) / rlo);
-                    ok = plo > 2e-5 + tol;
+                    // p_lo = (gam - 1) (elo - |m|^2 / (2 rlo)) > 2e-5 + tol, multiplied
+                    // through by rlo > 0 (no division; the margin over the 1e-5
+                    // floor dwarfs the rounding of either form)
+                    ok = (gam - 1.0) * (elo * rlo - 0.5 * (mx * mx + my * my)) > (2e-5 + tol) * rlo;
                 }
                 A.zsok[kp] = (uint8_t)((ok ? 1 : 0) | (cand ? 2 : 0));
             }
