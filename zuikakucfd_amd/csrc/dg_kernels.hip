@@ -667,29 +667,44 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
 // thread per (plane, ring cell) so that the launch fills the device.  It also
 // clears the counts of the limiter lists filled after it (z0, z1, or null),
 // which saves two single-thread launches per stage.
-__global__ void dg_ring_kernel(const double *__restrict__ Qn, const double *__restrict__ Qin,
-                               double *__restrict__ Qout, const int32_t *__restrict__ src,
-                               const int32_t *__restrict__ ring, int nring, size_t np, int nm, double ca,
-                               double cb, int stage, const StepScalars *__restrict__ S, double *__restrict__ M,
-                               int32_t *__restrict__ z0, int32_t *__restrict__ z1)
+struct RingArgs {
+    const double *Qn, *Qin;
+    double *Qout;
+    const int32_t *src, *ring;
+    int nring, nm;
+    size_t np;
+    double ca, cb;
+    int stage;
+    double *M;  // means of the limiter (mode 0 of every variable), or null
+};
+
+// Work item t of the ghost-ring combination: (plane, ring cell) pair t.
+__device__ __forceinline__ void ring_item(const RingArgs &R, size_t t)
+{
+    if (t >= (size_t)R.nring * 4 * R.nm) return;
+    const uint32_t tt = (uint32_t)t, p = tt / (uint32_t)R.nring;  // fewer than 2^32 (plane, ring cell) pairs
+    const size_t k = (size_t)R.ring[tt - p * (uint32_t)R.nring];
+    const int32_t e = R.src[k];
+    const size_t s = (size_t)(e >> kSrcShift);
+    const int var = (int)p / R.nm;
+    const double qn = R.Qn[p * R.np + k];
+    double qf = R.Qin[p * R.np + s];
+    if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
+    const double out = (R.stage == 0) ? qn + 0.0 : R.ca * qn + R.cb * (qf + 0.0);
+    R.Qout[p * R.np + k] = out;
+    if (R.M && (int)p % R.nm == 0) R.M[(size_t)var * R.np + k] = out;
+}
+
+__global__ void dg_ring_kernel(RingArgs R, const StepScalars *__restrict__ S, int32_t *__restrict__ z0,
+                               int32_t *__restrict__ z1)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         if (z0) *z0 = 0;
         if (z1) *z1 = 0;
     }
-    if (t >= (size_t)nring * 4 * nm || S->done) return;
-    const uint32_t tt = (uint32_t)t, p = tt / (uint32_t)nring;  // fewer than 2^32 (plane, ring cell) pairs
-    const size_t k = (size_t)ring[tt - p * (uint32_t)nring];
-    const int32_t e = src[k];
-    const size_t s = (size_t)(e >> kSrcShift);
-    const int var = (int)p / nm;
-    const double qn = Qn[p * np + k];
-    double qf = Qin[p * np + s];
-    if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
-    const double out = (stage == 0) ? qn + 0.0 : ca * qn + cb * (qf + 0.0);
-    Qout[p * np + k] = out;
-    if (M && (int)p % nm == 0) M[(size_t)var * np + k] = out;
+    if (S->done) return;
+    ring_item(R, t);
 }
 
 
@@ -1759,9 +1774,16 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
                                                           uint8_t *__restrict__ flags,
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
                                                           size_t ncell, size_t np, double Mh2, double gamma,
-                                                          const StepScalars *__restrict__ S)
+                                                          const StepScalars *__restrict__ S, RingArgs R,
+                                                          unsigned nscan)
 {
     if (S->done) return;  // uniform over the grid
+    if (blockIdx.x >= nscan) {
+        // blocks past the scan combine the ghost ring (when the limiter reads
+        // no ring cell, so the two parts are independent)
+        ring_item(R, (size_t)(blockIdx.x - nscan) * blockDim.x + threadIdx.x);
+        return;
+    }
     const int lane = threadIdx.x & 63, col = lane & 15;
     const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CW;
     if (t0 >= ncell) return;  // whole wave
@@ -2120,12 +2142,20 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         z0 = H.cand;
         if (A.zsok) z1 = H.tlist;
     }
-    if (H.nring > 0) {
+    const RingArgs RA{Qn, Qin, Qout, H.src, H.ring, H.nring, H.nm, H.g.np, A.ca, A.cb, stage, A.means};
+    const unsigned ring_blocks = (unsigned)(((size_t)H.nring * 4 * H.nm + 255) / 256);
+    // The ghost-ring combination writes only ring cells of Qout and of the
+    // means.  The TVB scan below changes updated cells and reads the means of
+    // their neighbours through the ghost map; unless that map reaches a ring
+    // cell (ring_read: fixed cells), the two are independent and the ring
+    // rides in the scan's launch as extra blocks.  With kernel timing on, or
+    // list counts to clear, it keeps its own launch.
+    const bool scan = ncell > 0 && H.cfg.limiter == ZK_LIMITER_TVB && A.zsok && !H.no_mfma;
+    const bool fold = scan && H.nring > 0 && !H.ring_read && !z0 && !z1 && !H.harvest && !H.prof;
+    if (H.nring > 0 && !fold) {
         ProfScope ps(H, "dg_ring");
-        const size_t nt = (size_t)H.nring * 4 * H.nm;
-        hipLaunchKernelGGL(dg_ring_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, H.stream, Qn, Qin, Qout,
-                           H.src, H.ring, H.nring, H.g.np, H.nm, A.ca, A.cb, stage, H.scal, A.means, z0, z1);
-    } else {
+        hipLaunchKernelGGL(dg_ring_kernel, dim3(ring_blocks), dim3(256), 0, H.stream, RA, H.scal, z0, z1);
+    } else if (H.nring == 0) {
         if (z0) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, z0);
         if (z1) hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, H.stream, z1);
     }
@@ -2148,10 +2178,11 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper
             constexpr int CW = P == 3 ? 32 : 64;
-            hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3((unsigned)((ncell + 4 * CW - 1) / (4 * CW))), dim3(256),
+            const unsigned nscan = (unsigned)((ncell + 4 * CW - 1) / (4 * CW));
+            hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256),
                                0, H.stream,
                                Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
-                               H.g.gamma, H.scal);
+                               H.g.gamma, H.scal, RA, nscan);
         } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
                                H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);

@@ -367,6 +367,20 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         }
     H->n_updated = nupd;
     H->nring = (int)ring.size();
+    // whether a neighbour of an updated cell takes its values from a ring cell
+    // (a fixed cell, or a ghost filled from one): then the DG limiter reads
+    // means the ring combination writes, and the two keep their order
+    H->ring_read = false;
+    auto in_ring = [&](size_t c) {
+        const size_t i = c / nx, j = c % nx;
+        return !(i >= 3 && i < (size_t)ny - 3 && j >= 3 && j < (size_t)nx - 3);
+    };
+    for (size_t k = 0; k < np && !H->ring_read; ++k) {
+        if (!upd[k]) continue;
+        const size_t nb[4] = {k - 1, k + 1, k - nx, k + nx};
+        for (size_t n : nb)
+            if (in_ring((size_t)(src[n] >> zk::kSrcShift))) H->ring_read = true;
+    }
 
     ZK_HIP(hipMalloc(&H->q, nst * sizeof(double)));
     ZK_HIP(hipMalloc(&H->qa, nst * sizeof(double)));

@@ -66,6 +66,7 @@ struct Handle {
     uint8_t *upd = nullptr;   // 1 = updated cell
     int32_t *ring = nullptr;  // indices of cells outside [3,ny-3)x[3,nx-3)
     int nring = 0;
+    bool ring_read = false;   // a neighbour of an updated cell takes its values from a ring cell
     double *gx = nullptr;     // DG x-face projected fluxes
     double *gy = nullptr;     // DG y-face projected fluxes
     double *means = nullptr;  // DG stage means (4 planes) for the TVB limiter
