@@ -1195,6 +1195,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 for (int v = 0; v < 4; ++v) Qr[2][v] = -Qr[2][v];
         };
         double Q[4][4];
+        bool upd;  // this lane's cell of row r is updated
         if (DMA) {
             // row r is complete in qbuf once every wave's loads have landed
             __builtin_amdgcn_s_waitcnt(0);
@@ -1203,6 +1204,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             // recomputed per row instead of being held (or spilled) across it
             const int x = 16 * wvu + (ln & 15);
             const int32_t eo = srcbuf[r % NS][x];
+            upd = (eo & kSrcUpdated) != 0;  // no global load (vmcnt counts in order)
             auto qb = qbuf[DB ? (r & 1) : 0];
             const int gq = (ln >> 4) <= P ? (ln >> 4) : 0;
 #pragma unroll
@@ -1226,6 +1228,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             }
         } else {
             load(r, Q);
+            upd = A.upd[k] != 0;
         }
         // face traces of this row: element side (W, E, N, S) at quadrature point g
         d4 Tr[4];
@@ -1324,7 +1327,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // U^n of row r, needed at the end of this iteration.  In the first
         // stage U^n is the row just read (an updated cell's ghost-map entry is
         // itself, unsigned), so only the cells that are not updated read it.
-        const bool upd = A.upd[k] != 0;
         double qn[4][4];
         if (store_col && slot && !(A.stage == 0 && upd))
 #pragma unroll

@@ -176,6 +176,11 @@ void build_src_map(const std::vector<int32_t> &bct, int ny, int nx, int slip, st
             if (bct[k] != -2) continue;
             src[k] = (int32_t)(((size_t)per(i, ny) * nx + per(j, nx)) << zk::kSrcShift);
         }
+    for (int i = 3; i < ny - 3; ++i)
+        for (int j = 3; j < nx - 3; ++j) {
+            const size_t k = (size_t)i * nx + j;
+            if (bct[k] == 0) src[k] |= zk::kSrcUpdated;
+        }
 }
 
 // The y-face map of the slip-wall option: the ghost map with the column

@@ -13,10 +13,13 @@ struct zk_mlp;
 
 namespace zk {
 
-// Ghost-fill map entry: (source cell index << 2) | negate bits.  Bit 0 negates
-// the x momentum, bit 1 the y momentum (slip ghosts).  A cell that is not
+// Ghost-fill map entry: (source cell index << 3) | flags.  Bit 0 negates the
+// x momentum, bit 1 the y momentum (slip ghosts); bit 2 marks an updated cell
+// (bctype 0 inside the update range), so that kernels which stage the map
+// rows in LDS have the flag without another global load.  A cell that is not
 // filled maps to itself.
-constexpr int kSrcShift = 2;
+constexpr int kSrcShift = 3;
+constexpr int32_t kSrcUpdated = 4;
 
 // Per-step scalars that stage kernels read from device memory, so that a
 // captured graph replays with a dt the device itself computed.
