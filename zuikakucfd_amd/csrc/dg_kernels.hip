@@ -1472,9 +1472,9 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
                     for (int v = 0; v <= P; ++v) {
-                        const double du = upd ? acc[c][v] * fdu[v] : 0.0;
-                        base[c][v] = (A.stage == 0) ? (upd ? Q[c][v] + dt * du : qn[c][v])
-                                                    : A.ca * qn[c][v] + A.cb * (Q[c][v] + dt * du);
+                        const double fa = fs[v] * acc[c][v];
+                        base[c][v] = (A.stage == 0) ? (upd ? fa + Q[c][v] : qn[c][v])
+                                                    : fa + (A.cb * Q[c][v] + A.ca * qn[c][v]);
                     }
             }
         } else if (store_col && slot) {
