@@ -330,6 +330,9 @@ def main() -> int:
     ms = s.step_timed(dt, args.steps)
     s.sync()
     rate = args.steps * 3 * nupd / (ms / 1e3)
+    # five more timed blocks of the same length: their median beside `value`
+    reps = sorted(s.step_timed(dt, args.steps) for _ in range(5))
+    rate_median5 = args.steps * 3 * nupd / (reps[2] / 1e3)
 
     # per-kernel durations with HIP events on the library's stream
     s.profile(True)
@@ -387,6 +390,8 @@ def main() -> int:
                      "kernel_avg_ms": avg_ms, "algo_bytes_per_cell_stage": bps,
                      "algo_bytes_per_launch": bps * nupd},
         "stage_hbm_frac": rate * bps / (HBM_PEAK_GBS * 1e9),
+        "median_of_5": {"value": rate_median5, "ms_per_step": reps[2] / args.steps,
+                        "note": "five further timed blocks of the same step count (HIP events)"},
         "kernels": rep,
         "finite": finite,
         "boundary": boundary,
