@@ -391,7 +391,8 @@ def main() -> int:
                      "algo_bytes_per_launch": bps * nupd},
         "stage_hbm_frac": rate * bps / (HBM_PEAK_GBS * 1e9),
         "median_of_5": {"value": rate_median5, "ms_per_step": reps[2] / args.steps,
-                        "note": "five further timed blocks of the same step count (HIP events)"},
+                        "note": "five further timed blocks of the same step count (HIP events), later in "
+                                "the same run, where the solution has more limited cells"},
         "kernels": rep,
         "finite": finite,
         "boundary": boundary,
