@@ -327,10 +327,9 @@ def main() -> int:
 
     import numpy as np
 
-    # Every timed block starts again from the initial data.  The DG P=3 TVB
-    # run at the fixed initial-CFL dt goes non-finite near step 185 (in the CPU
-    # oracle as well), so a block that continued from the previous one would
-    # time a blown-up solution.
+    # Every timed block starts again from the initial data, so all blocks time
+    # the same window of the solution (the count of limited cells grows with
+    # time) and none can run into a state the scheme has lost.
     def restart(first: bool = False) -> None:
         if not first:
             s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)

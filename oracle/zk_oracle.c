@@ -937,7 +937,15 @@ static inline double tvb_minmod(double a, double b, double c, double Mh2)
  * (px=1,py=0) and y-linear mode (px=0,py=1) are limited against forward and
  * backward differences of the neighbour means (neighbour means taken after
  * the ghost fill, y forward = +i); if either changes, every other mode >= 1
- * of that variable is zeroed and the cell is flagged. */
+ * of that variable is zeroed and the cell is flagged.  At P >= 3 the cell is
+ * also troubled when the one-dimensional TVB test (Cockburn & Shu 1989,
+ * Math. Comp. 52, sec. 2) changes one of its four face-average deviations:
+ * east sum_q q(q,0), west sum_q (-1)^(q+1) q(q,0), and the same over q(0,q)
+ * for the +i and -i faces, each against the same forward and backward mean
+ * differences as the linear mode of its axis.  The linear-mode test alone
+ * leaves a cell with small linear but large quadratic and cubic modes
+ * unlimited; at P = 3 that lets the four-quadrant problem blow up near
+ * t = 0.0054. */
 void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t *bct,
                       double *Q, uint8_t *flags)
 {
@@ -961,9 +969,25 @@ void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t 
                 double ub = a0[k];
                 double ax = Q[(size_t)(c * nm + mxm) * NP + k];
                 double ay = Q[(size_t)(c * nm + mym) * NP + k];
-                double axl = tvb_minmod(ax, a0[IDX(i, j + 1)] - ub, ub - a0[IDX(i, j - 1)], Mh2);
-                double ayl = tvb_minmod(ay, a0[IDX(i + 1, j)] - ub, ub - a0[IDX(i - 1, j)], Mh2);
-                if (axl != ax || ayl != ay) {
+                const double dxp = a0[IDX(i, j + 1)] - ub, dxm = ub - a0[IDX(i, j - 1)];
+                const double dyp = a0[IDX(i + 1, j)] - ub, dym = ub - a0[IDX(i - 1, j)];
+                double axl = tvb_minmod(ax, dxp, dxm, Mh2);
+                double ayl = tvb_minmod(ay, dyp, dym, Mh2);
+                int trouble = axl != ax || ayl != ay;
+                if (P >= 3) {
+                    double e = 0.0, w = 0.0, n = 0.0, so = 0.0;
+                    for (int q = 1; q <= P; ++q) {
+                        const double qx = Q[(size_t)(c * nm + q * (P + 1)) * NP + k];
+                        const double qy = Q[(size_t)(c * nm + q) * NP + k];
+                        e += qx;
+                        n += qy;
+                        if (q & 1) { w += qx; so += qy; }
+                        else { w -= qx; so -= qy; }
+                    }
+                    trouble = trouble || tvb_minmod(e, dxp, dxm, Mh2) != e || tvb_minmod(w, dxp, dxm, Mh2) != w ||
+                              tvb_minmod(n, dyp, dym, Mh2) != n || tvb_minmod(so, dyp, dym, Mh2) != so;
+                }
+                if (trouble) {
                     for (int m = 1; m < nm; ++m) Q[(size_t)(c * nm + m) * NP + k] = 0.0;
                     Q[(size_t)(c * nm + mxm) * NP + k] = axl;
                     Q[(size_t)(c * nm + mym) * NP + k] = ayl;

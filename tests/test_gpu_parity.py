@@ -445,6 +445,41 @@ def test_weno_slip_walls_channel_positivity():
         assert rel_l1(U[c], o.U[c]) <= TOL, c
 
 
+def test_dg3_tvb_face_test_parity_100_steps():
+    """P = 3 TVB (with the face-average test) on the four-quadrant problem at
+    64^2 over 100 steps at the DG CFL: state and limiter flags against the
+    oracle, through the fused stage kernel and the TVB scan."""
+    say("dg P=3 tvb 64^2 x100 (face-average test)")
+    m = meshio.riemann_2d(64)
+    s, o = dg_pair(m, 3, zk.LIMITER_TVB, M=50.0)
+    dt = oracle.dg_cfl_limit(3) * m.da / o.max_speed()
+    s.step(dt, 100)
+    o.step(dt, 100)
+    Q = s.get_state()
+    assert np.isfinite(Q).all()
+    for c in range(4):
+        assert rel_l1(Q[c], o.Q[c]) <= TOL, c
+    np.testing.assert_array_equal(s.get_flags(), o.flags)
+
+
+def test_dg3_tvb_riemann256_stays_finite():
+    """With the linear-mode test alone, P = 3 TVB on the four-quadrant problem
+    goes non-finite near step 190 at 256^2 (and at 1024^2), in the oracle as
+    on the GPU.  With the face-average test it stays finite for 400 steps
+    (t = 0.045), with positive means of density and energy."""
+    say("dg P=3 tvb 256^2 x400 finite")
+    m = meshio.riemann_2d(256)
+    s = zk.Solver(method=zk.METHOD_DG, P=3, limiter=zk.LIMITER_TVB, tvb_M=50.0)
+    s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    dt = oracle.dg_cfl_limit(3) * m.da / s.max_speed()
+    for _ in range(4):
+        s.step(dt, 100)
+        U = s.get_conserved()
+        assert np.isfinite(U).all()
+    upd = updated_mask(m.bctype)
+    assert U[0][upd].min() > 0 and U[3][upd].min() > 0
+
+
 @pytest.mark.parametrize("P", [1, 2, 3])
 def test_dg_tvb_low_pressure(P):
     """Pressure 2e-5 under a compressing and expanding periodic flow: the

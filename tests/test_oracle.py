@@ -458,6 +458,52 @@ def test_tvb_limiter_closed_forms():
     assert np.all(o3.Q[:, 1:, 6, 5] == 0.0)
 
 
+def test_tvb_face_average_test_at_p3():
+    """At P >= 3 the TVB test also checks the face-average deviations
+    E = sum_q a(q,0), W = sum_q (-1)^(q+1) a(q,0) (and N, S over a(0,q))
+    against the neighbour mean differences.  On a ramp with differences
+    d = 0.01 in x and 0 in y (M = 0):
+    - a1 = 0.008, a2 = -0.004: E = 0.004 passes, W = 0.012 > d -> limited
+      (a sign slip in W, W = a1 + a2, would pass it);
+    - a1 = 0.008, a2 = +0.001: E = 0.009, W = 0.007 -> untouched;
+    - a(0,2) = 1e-3 with no y differences: N = 1e-3 -> limited (y axis);
+    - at P = 2 the same modes leave the first cell alone (linear test only)."""
+    n, h = 14, 0.1
+    bct = meshio.ring_bctype(n, n)
+    j = np.arange(n)
+
+    def ramp(P):
+        nq = P + 1
+        Q = np.zeros((4, nq * nq, n, n))
+        Q[:, 0] = 1.0 + 0.01 * j[None, None, :]
+        Q[:, nq] = 0.005  # x-linear coefficient of the ramp
+        return Q
+
+    P = 3
+    Q = ramp(P)
+    Q[:, 4, 6, 6], Q[:, 8, 6, 6] = 0.008, -0.004  # modes (1,0), (2,0)
+    Q[:, 4, 6, 8], Q[:, 8, 6, 8] = 0.008, 0.001
+    Q[:, 2, 4, 6] = 1e-3  # mode (0,2)
+    o = oracle.DG(P, h, G, Q, bct, tvb_M=0.0)
+    f = o.limit_tvb()
+    inner = np.zeros_like(f, dtype=bool)
+    inner[3:n - 3, 4:n - 4] = True
+    assert sorted(zip(*np.nonzero(f & inner))) == [(4, 6), (6, 6)]
+    # limited: modes >= 1 zeroed, the x-linear mode kept by minmod(0.008, d, d)
+    assert np.all(o.Q[:, 4, 6, 6] == 0.008) and np.all(o.Q[:, 8, 6, 6] == 0.0)
+    assert np.all(o.Q[:, 2, 4, 6] == 0.0) and np.all(o.Q[:, 4, 4, 6] == 0.005)
+    np.testing.assert_array_equal(o.Q[:, :, 6, 8], Q[:, :, 6, 8])
+    # the face test is off at P = 2
+    Q2 = ramp(2)
+    Q2[:, 3, 6, 6], Q2[:, 6, 6, 6] = 0.008, -0.004  # modes (1,0), (2,0)
+    o2 = oracle.DG(2, h, G, Q2, bct, tvb_M=0.0)
+    f2 = o2.limit_tvb()
+    assert not (f2 & inner).any()
+    # with M h^2 above every deviation nothing is limited at P = 3 either
+    o3 = oracle.DG(P, h, G, Q, bct, tvb_M=2.0)
+    assert o3.limit_tvb().sum() == 0
+
+
 def test_persson_closed_forms():
     P, n = 2, 8
     nm = 9

@@ -233,7 +233,8 @@ struct DgArgs {
     double *means;    // stage means (4 planes) for the TVB limiter, or null
     // fused kernel under the TVB limiters, or null: bit 0 set where the
     // positivity scaling is certified to be a no-op, bit 1 where the cell is
-    // a TVB candidate (a linear mode above Mh2, or NaN)
+    // a TVB candidate (a linear mode above Mh2, or NaN; at P >= 3 a mode of
+    // degree 0 on one axis above Mh2 / 4)
     uint8_t *zsok;
     double Mh2;
     int ny, nx, nxu, nyu;
@@ -576,8 +577,85 @@ __device__ __forceinline__ double filled_mean(const double *__restrict__ M, cons
     return v;
 }
 
+// TVB-modified minmod: a when |a| <= M h^2, else minmod(a, b, c) (NaN goes
+// to the minmod, which returns 0).
+__device__ __forceinline__ double tvb_mod(double a, double b, double c, double Mh2)
+{
+    return fabs(a) <= Mh2 ? a : minmod3(a, b, c);
+}
+
+// Face-average deviations of one variable along one axis from its modes
+// q[0..P-1] of degree 1..P on that axis (degree 0 on the other): the + face
+// average minus the mean, sum q, and the mean minus the - face average,
+// sum (-1)^(deg+1) q, each summed in increasing degree.
+template <int P>
+__device__ __forceinline__ void face_devs(const double q[P], double &fp, double &fm)
+{
+    fp = 0.0;
+    fm = 0.0;
+#pragma unroll
+    for (int d = 1; d <= P; ++d) {
+        fp += q[d - 1];
+        if (d & 1)
+            fm += q[d - 1];
+        else
+            fm -= q[d - 1];
+    }
+}
+
+// TVB test of variable c of cell k (Cockburn & Shu 2001, modified minmod):
+// the linear modes ax (x) and ay (y) against the forward and backward
+// differences of the filled neighbour means; with faces (P >= 3) the variable
+// is also troubled when the one-dimensional test (Cockburn & Shu 1989)
+// changes one of the face-average deviations xp, xm, yp, ym.  Leaves the
+// limited linear modes in axl/ayl and returns whether the variable is
+// troubled.  Neighbour means are read only where a value is above M h^2.
+__device__ __forceinline__ bool tvb_axes(double ub, double ax, double ay, bool faces, double xp, double xm,
+                                         double yp, double ym, const double *__restrict__ M,
+                                         const int32_t *__restrict__ src, size_t np, size_t k, int nx, int c,
+                                         double Mh2, double &axl, double &ayl)
+{
+    bool trouble = false;
+    axl = ax;
+    ayl = ay;
+    if (!(fabs(ax) <= Mh2) || (faces && (!(fabs(xp) <= Mh2) || !(fabs(xm) <= Mh2)))) {
+        const double dp = filled_mean(M, src, np, k + 1, c) - ub, dm = ub - filled_mean(M, src, np, k - 1, c);
+        axl = tvb_mod(ax, dp, dm, Mh2);
+        if (faces) trouble = tvb_mod(xp, dp, dm, Mh2) != xp || tvb_mod(xm, dp, dm, Mh2) != xm;
+    }
+    if (!(fabs(ay) <= Mh2) || (faces && (!(fabs(yp) <= Mh2) || !(fabs(ym) <= Mh2)))) {
+        const double dp = filled_mean(M, src, np, k + nx, c) - ub, dm = ub - filled_mean(M, src, np, k - nx, c);
+        ayl = tvb_mod(ay, dp, dm, Mh2);
+        if (faces) trouble = trouble || tvb_mod(yp, dp, dm, Mh2) != yp || tvb_mod(ym, dp, dm, Mh2) != ym;
+    }
+    return trouble || axl != ax || ayl != ay;
+}
+
+// tvb_axes on variable c of a cell held as C[c][px * (P + 1) + py].
+template <int P>
+__device__ __forceinline__ bool tvb_var(const double Cc[(P + 1) * (P + 1)], const double *__restrict__ M,
+                                        const int32_t *__restrict__ src, size_t np, size_t k, int nx, int c,
+                                        double Mh2, double &axl, double &ayl)
+{
+    constexpr int NQ = P + 1;
+    const double ax = Cc[NQ], ay = Cc[1];
+    double xp = ax, xm = ax, yp = ay, ym = ay;
+    if constexpr (P >= 3) {
+        double qx[P], qy[P];
+#pragma unroll
+        for (int d = 1; d <= P; ++d) {
+            qx[d - 1] = Cc[d * NQ];
+            qy[d - 1] = Cc[d];
+        }
+        face_devs<P>(qx, xp, xm);
+        face_devs<P>(qy, yp, ym);
+    }
+    return tvb_axes(Cc[0], ax, ay, P >= 3, xp, xm, yp, ym, M, src, np, k, nx, c, Mh2, axl, ayl);
+}
+
 // Component-wise TVB limiter (Cockburn & Shu 2001, modified minmod with
-// constant M) then the positivity scaling, in place on the stage output.
+// constant M; at P >= 3 also the face-average test of tvb_var) then the
+// positivity scaling, in place on the stage output.
 // Neighbour means are the combined stage means after the ghost fill; the cell
 // is written back only when the limiter or the scaling changed it.
 template <int P>
@@ -602,14 +680,8 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
         constexpr int mxm = NQ, mym = 1;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double ub = C[c][0];
-            const double ax = C[c][mxm], ay = C[c][mym];
-            double axl = ax, ayl = ay;
-            if (!(fabs(ax) <= Mh2))
-                axl = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
-            if (!(fabs(ay) <= Mh2))
-                ayl = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
-            if (axl != ax || ayl != ay) {
+            double axl, ayl;
+            if (tvb_var<P>(C[c], M, src, np, k, nx, c, Mh2, axl, ayl)) {
 #pragma unroll
                 for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
                 C[c][mxm] = axl;
@@ -1278,7 +1350,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         }
         // sab[c]: sum over the non-mean modes of |mode| * max |basis| at the positivity points
         double sab[4] = {0.0, 0.0, 0.0, 0.0}, mean0[4] = {0.0, 0.0, 0.0, 0.0};
-        bool lin_big = false;  // this lane's linear mode (x in group 0, y in group 1) is above M h^2, or NaN
+        // this lane's linear mode (x in group 0, y in group 1) is above M h^2, or
+        // NaN; at P >= 3 any of its modes of degree 0 on one axis is above M h^2 / 4
+        // (the face-average deviations sum P = 3 of them, so a cell the TVB test
+        // can change is always marked)
+        bool lin_big = false;
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
@@ -1293,7 +1369,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     } else {
                         sab[c] += fabs(out) * (F4 ? M4.zc[4 * v + g] : M.zc[4 * v + g]);
                     }
-                    if ((v == 1 && g == 0) || (v == 0 && g == 1)) lin_big = lin_big || !(fabs(out) <= A.Mh2);
+                    if (P >= 3 ? ((g == 0 && v >= 1) || (v == 0 && g >= 1)) : ((v == 1 && g == 0) || (v == 0 && g == 1)))
+                        lin_big = lin_big || !((P >= 3 ? 4.0 * fabs(out) : fabs(out)) <= A.Mh2);
                 }
         }
         if (A.zsok && r > r0) {
@@ -1307,6 +1384,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 sab[c] += __shfl_down(sab[c], 16);
             }
             int cand = lin_big ? 1 : 0;
+            if (P >= 3) cand |= __shfl_down(cand, 32);  // groups 2 and 3 onto 0 and 1
             cand |= __shfl_down(cand, 16);  // group 1's y-linear test
             if (g == 0 && store_col) {
                 const double rlo = mean0[0] - sab[0];
@@ -1496,26 +1574,19 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     }
 }
 
-// The TVB test of one cell (per variable: modified minmod of the two linear
-// modes against the filled neighbour means); returns whether any variable
-// changes and leaves the limited slopes in axl/ayl.
+// The TVB test of one cell (per variable: tvb_var, i.e. the two linear modes
+// and, at P >= 3, the face-average deviations, against the filled neighbour
+// means); returns whether any variable is troubled and leaves the limited
+// slopes in axl/ayl.
 template <int P>
 __device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], const double *__restrict__ M,
                                          const int32_t *__restrict__ src, size_t np, size_t k, int nx, double Mh2,
                                          double axl[4], double ayl[4], bool chg[4])
 {
-    constexpr int NQ = P + 1;
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const double ub = C[c][0], ax = C[c][NQ], ay = C[c][1];
-        axl[c] = ax;
-        ayl[c] = ay;
-        if (!(fabs(ax) <= Mh2))
-            axl[c] = minmod3(ax, filled_mean(M, src, np, k + 1, c) - ub, ub - filled_mean(M, src, np, k - 1, c));
-        if (!(fabs(ay) <= Mh2))
-            ayl[c] = minmod3(ay, filled_mean(M, src, np, k + nx, c) - ub, ub - filled_mean(M, src, np, k - nx, c));
-        chg[c] = (axl[c] != ax || ayl[c] != ay);
+        chg[c] = tvb_var<P>(C[c], M, src, np, k, nx, c, Mh2, axl[c], ayl[c]);
         any = any || chg[c];
     }
     return any;
@@ -1525,13 +1596,14 @@ __device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], c
 // kernel: a wave owns 16 consecutive cells of the update range, lane (g, col)
 // holds modes (px = v, py = g) of cell col.  The TVB test of a cell runs in its
 // lane of group 0 (mean and x-linear mode there, the y-linear mode shuffled in
-// from group 1) and its outcome is broadcast down the column; the positivity
+// from group 1, and at P >= 3 the (0,q) modes from group q for the face-average
+// deviations) and its outcome is broadcast down the column; the positivity
 // scaling evaluates rho and then all variables at the 4x4 points with MFMA
 // products and takes the minimum theta across the column's lanes.  Same
 // results as dg_tvb_kernel up to the summation order of the point values.
 //
 // With zsok (written by the fused stage kernel) the kernel first loads only
-// what the TVB test reads (means and linear modes); a wave whose cells are
+// what the TVB test reads (means and the modes of degree 0 on one axis); a wave whose cells are
 // all unlimited and certified positive by zsok ends there, and only the other
 // waves load every mode for the limiting and the positivity scaling.
 // The TVB + positivity work of the 16 cells of one wave (cell k of column
@@ -1549,14 +1621,17 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     const bool slot = g <= P;
     const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
     double C[4][4];
-    // phase 1: the modes the TVB test reads, (0,0), (1,0) in group 0 and (0,1) in group 1
+    // Modes the TVB test reads: (0,0), (1,0) in group 0 and (0,1) in group 1;
+    // at P >= 3 every mode (q,0) in group 0 and every mode (0,q) in group q.
+    auto tvb_reads = [g](int v) {
+        return P >= 3 ? ((g == 0 && v <= P) || (v == 0 && g <= P)) : ((g == 0 && v <= 1) || (g == 1 && v == 0));
+    };
+    // phase 1: the modes the TVB test reads
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const bool need = (g == 0 && v <= 1) || (g == 1 && v == 0);
-            C[c][v] = need ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
-        }
+        for (int v = 0; v < 4; ++v)
+            C[c][v] = tvb_reads(v) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
 
     // ---- TVB (cells with bctype 0) ----
     int chg = 0;  // bit c: variable c limited
@@ -1566,13 +1641,17 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     for (int c = 0; c < 4; ++c) {
         const double ay_in = __shfl(C[c][0], 16 + col);   // mode (0, 1) of this column's cell
         const double ub = C[c][0], ax = C[c][1];          // in group 0: modes (0, 0) and (1, 0)
+        double xp = ax, xm = ax, yp = ay_in, ym = ay_in;
+        if constexpr (P >= 3) {
+            // face-average deviations: modes (q, 0) in this lane, (0, q) from group q
+            const double qx[3] = {C[c][1], C[c][2], C[c][3]};
+            const double qy[3] = {ay_in, __shfl(C[c][0], 32 + col), __shfl(C[c][0], 48 + col)};
+            face_devs<3>(qx, xp, xm);
+            face_devs<3>(qy, yp, ym);
+        }
         double a_x = ax, a_y = ay_in;
         if (g == 0 && tvb_cell) {
-            if (!(fabs(ax) <= Mh2))
-                a_x = minmod3(ax, filled_mean(Mn, src, np, k + 1, c) - ub, ub - filled_mean(Mn, src, np, k - 1, c));
-            if (!(fabs(ay_in) <= Mh2))
-                a_y = minmod3(ay_in, filled_mean(Mn, src, np, k + nx, c) - ub, ub - filled_mean(Mn, src, np, k - nx, c));
-            if (a_x != ax || a_y != ay_in) chg |= 1 << c;
+            if (tvb_axes(ub, ax, ay_in, P >= 3, xp, xm, yp, ym, Mn, src, np, k, nx, c, Mh2, a_x, a_y)) chg |= 1 << c;
         }
         axl[c] = __shfl(a_x, col);
         ayl[c] = __shfl(a_y, col);
@@ -1591,8 +1670,7 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const bool had = (g == 0 && v <= 1) || (g == 1 && v == 0);
-            if (!had && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+            if (!tvb_reads(v) && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
         }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
@@ -1825,13 +1903,22 @@ __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c)
 // A cell can only be changed by the TVB test when one of its linear modes is
 // above M h^2 (the minmod is skipped otherwise); such cells are the
 // candidates the network is asked about.  NaN counts as a candidate, as the
-// test treats it.
+// test treats it.  At P >= 3 the face-average deviations sum P = 3 modes of
+// degree 0 on one axis, so a candidate has one of those above M h^2 / 4.
 template <int P>
 __device__ __forceinline__ bool tvb_candidate(const double C[4][(P + 1) * (P + 1)], double Mh2)
 {
     bool c = false;
+    if constexpr (P >= 3) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) c = c || !(fabs(C[v][P + 1]) <= Mh2) || !(fabs(C[v][1]) <= Mh2);
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int d = 1; d <= P; ++d)
+                c = c || !(4.0 * fabs(C[v][d * (P + 1)]) <= Mh2) || !(4.0 * fabs(C[v][d]) <= Mh2);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c = c || !(fabs(C[v][P + 1]) <= Mh2) || !(fabs(C[v][1]) <= Mh2);
+    }
     return c;
 }
 
@@ -1856,10 +1943,12 @@ __global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__
         if (bct[k] == 0) {
             double C[4][NM];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
-                C[c][P + 1] = Q[(size_t)(c * NM + P + 1) * np + k];
-            }
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int d = 1; d <= (P >= 3 ? P : 1); ++d) {
+                    C[c][d] = Q[(size_t)(c * NM + d) * np + k];
+                    C[c][d * (P + 1)] = Q[(size_t)(c * NM + d * (P + 1)) * np + k];
+                }
             is = tvb_candidate<P>(C, Mh2);
         }
     }
