@@ -233,8 +233,8 @@ struct DgArgs {
     double *means;    // stage means (4 planes) for the TVB limiter, or null
     // fused kernel under the TVB limiters, or null: bit 0 set where the
     // positivity scaling is certified to be a no-op, bit 1 where the cell is
-    // a TVB candidate (a linear mode above Mh2, or NaN; at P >= 3 a mode of
-    // degree 0 on one axis above Mh2 / 4)
+    // a TVB candidate (a linear mode above Mh2, or NaN; at P >= 3 a variable
+    // with sum_q |a(q,0)| above Mh2 or some |a(0,q)| above Mh2 / 4)
     uint8_t *zsok;
     double Mh2;
     int ny, nx, nxu, nyu;
@@ -1351,14 +1351,18 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // sab[c]: sum over the non-mean modes of |mode| * max |basis| at the positivity points
         double sab[4] = {0.0, 0.0, 0.0, 0.0}, mean0[4] = {0.0, 0.0, 0.0, 0.0};
         // this lane's linear mode (x in group 0, y in group 1) is above M h^2, or
-        // NaN; at P >= 3 any of its modes of degree 0 on one axis is above M h^2 / 4
-        // (the face-average deviations sum P = 3 of them, so a cell the TVB test
-        // can change is always marked)
+        // NaN.  At P >= 3 the TVB test also reads the face-average deviations,
+        // sums of the P = 3 modes of degree 0 on one axis: group 0 marks a
+        // variable whose sum_q |a(q,0)| is above M h^2 (a bound of the x
+        // deviations, exact under monotone rounding), group q one whose |a(0,q)|
+        // is above M h^2 / 4 (the y deviations then stay below 3/4 M h^2).  A
+        // cell the TVB test can change is always marked.
         bool lin_big = false;
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < 4; ++c) {
+                double t0 = 0.0, sx = 0.0;  // P >= 3: |a(0,g)| and sum_q |a(q,0)| of this lane
 #pragma unroll
                 for (int v = 0; v <= P; ++v) {
                     const double out = base[c][v] - fs[v] * gN[c][v];
@@ -1369,9 +1373,17 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     } else {
                         sab[c] += fabs(out) * (F4 ? M4.zc[4 * v + g] : M.zc[4 * v + g]);
                     }
-                    if (P >= 3 ? ((g == 0 && v >= 1) || (v == 0 && g >= 1)) : ((v == 1 && g == 0) || (v == 0 && g == 1)))
-                        lin_big = lin_big || !((P >= 3 ? 4.0 * fabs(out) : fabs(out)) <= A.Mh2);
+                    if constexpr (P >= 3) {
+                        if (v == 0)
+                            t0 = out;
+                        else
+                            sx += fabs(out);
+                    } else if ((v == 1 && g == 0) || (v == 0 && g == 1)) {
+                        lin_big = lin_big || !(fabs(out) <= A.Mh2);
+                    }
                 }
+                if constexpr (P >= 3) lin_big = lin_big || !((g == 0 ? sx : 4.0 * fabs(t0)) <= A.Mh2);
+            }
         }
         if (A.zsok && r > r0) {
             // Sufficient condition for the positivity scaling to leave the cell
@@ -1904,17 +1916,23 @@ __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c)
 // above M h^2 (the minmod is skipped otherwise); such cells are the
 // candidates the network is asked about.  NaN counts as a candidate, as the
 // test treats it.  At P >= 3 the face-average deviations sum P = 3 modes of
-// degree 0 on one axis, so a candidate has one of those above M h^2 / 4.
+// degree 0 on one axis, so a candidate has, for some variable, the sum of
+// |a(q,0)| above M h^2 or some |a(0,q)| above M h^2 / 4.
 template <int P>
 __device__ __forceinline__ bool tvb_candidate(const double C[4][(P + 1) * (P + 1)], double Mh2)
 {
     bool c = false;
     if constexpr (P >= 3) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < 4; ++v) {
+            double sx = 0.0;
 #pragma unroll
-            for (int d = 1; d <= P; ++d)
-                c = c || !(4.0 * fabs(C[v][d * (P + 1)]) <= Mh2) || !(4.0 * fabs(C[v][d]) <= Mh2);
+            for (int d = 1; d <= P; ++d) {
+                sx += fabs(C[v][d * (P + 1)]);
+                c = c || !(4.0 * fabs(C[v][d]) <= Mh2);
+            }
+            c = c || !(sx <= Mh2);
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < 4; ++v) c = c || !(fabs(C[v][P + 1]) <= Mh2) || !(fabs(C[v][1]) <= Mh2);
