@@ -49,7 +49,8 @@ typedef int zk_status;
 /* DG limiter */
 #define ZK_LIMITER_NONE 0
 #define ZK_LIMITER_PERSSON_ZS 1 /* applyLimiter, src_DG/fluid.cpp:824-992 */
-#define ZK_LIMITER_TVB 2        /* Cockburn-Shu TVB (M) + Zhang-Shu positivity */
+#define ZK_LIMITER_TVB 2        /* Cockburn-Shu TVB (M; at P >= 3 also the face-average
+                                   test) + Zhang-Shu positivity */
 #define ZK_LIMITER_LEARNED 3    /* MLP troubled-cell indicator gating the TVB limiter */
 
 typedef struct zk_config {
