@@ -504,6 +504,21 @@ def test_tvb_face_average_test_at_p3():
     assert o3.limit_tvb().sum() == 0
 
 
+def test_dg3_tvb_four_quadrant_stays_finite():
+    """The four-quadrant problem at 48^2, DG P=3 with TVB (M = 50) at the DG
+    CFL, 300 steps.  With the linear-mode test alone the oracle goes non-finite
+    between steps 240 and 250 here (and near step 190 at 256^2 and 1024^2);
+    the face-average test keeps every value finite and the means of density
+    and energy positive."""
+    m = meshio.riemann_2d(48)
+    o = oracle.DG.from_prims(3, m.da, G, m.rho, m.u, m.v, m.p, m.bctype, limiter=2, tvb_M=50.0)
+    dt = oracle.dg_cfl_limit(3) * m.da / o.max_speed()
+    o.step(dt, 300)
+    assert np.isfinite(o.Q).all()
+    upd = updated_mask(m.bctype)
+    assert o.Q[0, 0][upd].min() > 0 and o.Q[3, 0][upd].min() > 0
+
+
 def test_persson_closed_forms():
     P, n = 2, 8
     nm = 9
