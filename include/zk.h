@@ -34,8 +34,19 @@ typedef int zk_status;
 #define ZK_OK 0
 #define ZK_ERR_ARG 1      /* bad argument or call order */
 #define ZK_ERR_HIP 2      /* HIP runtime error */
-#define ZK_ERR_NUMERIC 3  /* non-finite value detected */
+#define ZK_ERR_NUMERIC 3  /* non-finite state: the message names the variable, the cell
+                             (i, j) and the step count of the check that found it */
 #define ZK_ERR_CFL 4      /* DG CFL limit exceeded (src_DG/solver.cpp:245-260) */
+
+/* Failure detection (no reference counterpart; the reference keeps stepping
+ * and writing NaN files).  Every zk_step, zk_step_cfl, zk_load and
+ * zk_set_state call ends by enqueueing a check of the state's cell means
+ * (mode 0 of U0..U3, every cell).  The calls that wait for the stream --
+ * zk_sync, zk_step_timed, zk_step_cfl, zk_get_state and zk_get_conserved (which
+ * still fill their outputs) -- return ZK_ERR_NUMERIC when a check found a
+ * value that is not finite, with the variable, the cell and the step count of
+ * that check in zk_last_error.  zk_step and zk_step_cfl then refuse to step
+ * (ZK_ERR_NUMERIC) until zk_load or zk_set_state replaces the state. */
 
 /* method */
 #define ZK_METHOD_WENO 0  /* src_WENO: WENO5 FV + HLLC, SSP-RK3 */
@@ -115,7 +126,10 @@ zk_status zk_step(zk_handle *h, double dt, int nsteps);
  * device before every step (lambda from zk_max_speed's rule), clipped so that
  * time never passes t_end; stops early at t_end.  The time and step count
  * reached are returned.  The CFL-correct replacement of the `auto`/`time`
- * dt rule of src_WENO/solver.cpp:148-171, which sets dt = lambda * h. */
+ * dt rule of src_WENO/solver.cpp:148-171, which sets dt = lambda * h.
+ * DG: a cfl above the stable limit 0.9 / (2 (2P + 1)) returns ZK_ERR_CFL, as
+ * the reference refuses to run (src_DG/solver.cpp:12-17, 245-260).  A wave
+ * speed that is not finite stops the stepping and returns ZK_ERR_NUMERIC. */
 zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps,
                       double *t_out, int *steps_out);
 
@@ -152,16 +166,21 @@ zk_status zk_updated_cells(zk_handle *h, int64_t *out);
 typedef struct zk_ensemble zk_ensemble;
 
 zk_status zk_ensemble_create(const zk_config *cfg, zk_ensemble **out);
-/* Append one instance (primitive fields, as zk_load); copied. */
+/* Append one instance (primitive fields, as zk_load); copied.  Refused
+ * (ZK_ERR_ARG) once the ensemble is finalized. */
 zk_status zk_ensemble_add(zk_ensemble *e, int ny, int nx, double da, double gamma,
                           const double *rho, const double *u, const double *v,
                           const double *p, const int32_t *bctype, double cfl);
-/* Upload every instance; no zk_ensemble_add afterwards. */
+/* Upload every instance; no zk_ensemble_add afterwards.  A second call
+ * returns ZK_ERR_ARG and changes nothing. */
 zk_status zk_ensemble_finalize(zk_ensemble *e);
 /* tile: 1 = 16x15, 2 = 32x15 cells per work item; use_graph: replay a captured step. */
 zk_status zk_ensemble_set_tuning(zk_ensemble *e, int tile, int use_graph);
 /* Advance every instance nsteps steps (asynchronous). */
 zk_status zk_ensemble_step(zk_ensemble *e, int nsteps);
+/* Wait for the ensemble's stream.  An instance whose wave-speed maximum was
+ * not finite stops (dt = 0 from then on); sync then returns ZK_ERR_NUMERIC
+ * naming the first such instances and their step counts. */
 zk_status zk_ensemble_sync(zk_ensemble *e);
 zk_status zk_ensemble_size(zk_ensemble *e, int *ninst, int64_t *updated_cells);
 /* Conserved planes (4*ny*nx) and time reached of instance idx. */

@@ -729,7 +729,7 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
             const double u = U[1] / rho, v = U[2] / rho;
             const double p = smax((gam - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v)), eps);
             const double a = sqrt(gam * p / rho);
-            mx = smax(mx, sqrt(u * u + v * v) + a);
+            mx = speed_max(mx, sqrt(u * u + v * v) + a, finite4(U[0], U[1], U[2], U[3]));
         }
     }
     block_max_to(mx, out);

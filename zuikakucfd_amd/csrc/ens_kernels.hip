@@ -151,8 +151,7 @@ __global__ __launch_bounds__(256) void ens_max_speed_kernel(EnsArgs E)
         const double u = U[np + k] / rho, v = U[2 * np + k] / rho;
         const double p = (gamma - 1.0) * (U[3 * np + k] - 0.5 * rho * (u * u + v * v));
         const double a = sqrt(gamma * p / rho);
-        const double spd = sqrt(u * u + v * v) + a;
-        if (spd > m) m = spd;
+        m = zk::speed_max(m, sqrt(u * u + v * v) + a, zk::finite4(rho, U[np + k], U[2 * np + k], U[3 * np + k]));
     }
     zk::block_max_to(m, E.lam + ch.x);
 }
@@ -164,14 +163,21 @@ __global__ void ens_dt_kernel(EnsArgs E)
     const double lam = __longlong_as_double((long long)E.lam[i]);
     StepScalars &S = E.S[i];
     S.lambda = lam;
-    S.dt = E.inst[i].cfl * E.inst[i].da / lam;
     S.done = 0;
+    if (S.bad || !(lam <= 1.79769313486231570815e308)) {
+        // a state that is no longer finite (or a zero density): the instance
+        // stops with dt = 0 and zk_ensemble_sync reports it
+        S.bad = 1;
+        S.dt = 0.0;
+        return;
+    }
+    S.dt = E.inst[i].cfl * E.inst[i].da / lam;
 }
 
 __global__ void ens_advance_kernel(EnsArgs E)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E.ninst) return;
+    if (i >= E.ninst || E.S[i].bad) return;  // a stopped instance keeps its time and step count
     E.S[i].t += E.S[i].dt;
     E.S[i].steps += 1;
 }
@@ -375,7 +381,15 @@ zk_status zk_ensemble_set_tuning(zk_ensemble *E, int tile, int use_graph)
 
 zk_status zk_ensemble_finalize(zk_ensemble *E)
 {
-    if (!E || E->inst.empty()) return ZK_ERR_ARG;
+    if (!E) return ZK_ERR_ARG;
+    if (E->final) {
+        E->err = "zk_ensemble_finalize: the ensemble is already finalized";
+        return ZK_ERR_ARG;
+    }
+    if (E->inst.empty()) {
+        E->err = "zk_ensemble_finalize: no instance added";
+        return ZK_ERR_ARG;
+    }
     ENS_HIP(hipSetDevice(E->cfg.device));
     E->final = true;
     const size_t ns = E->h_state.size(), nc = E->h_src.size();
@@ -455,8 +469,22 @@ zk_status zk_ensemble_step(zk_ensemble *E, int nsteps)
 zk_status zk_ensemble_sync(zk_ensemble *E)
 {
     if (!E) return ZK_ERR_ARG;
+    ENS_HIP(hipSetDevice(E->cfg.device));
     ENS_HIP(hipStreamSynchronize(E->stream));
-    return ZK_OK;
+    if (!E->final) return ZK_OK;
+    // instances whose wave-speed maximum was not finite have stopped
+    std::vector<StepScalars> s(E->inst.size());
+    ENS_HIP(hipMemcpy(s.data(), E->S, s.size() * sizeof(StepScalars), hipMemcpyDeviceToHost));
+    int nbad = 0;
+    std::string which;
+    for (size_t k = 0; k < s.size(); ++k)
+        if (s[k].bad) {
+            if (nbad < 8) which += (nbad ? ", " : "") + std::to_string(k) + " (step " + std::to_string(s[k].steps) + ")";
+            ++nbad;
+        }
+    if (!nbad) return ZK_OK;
+    E->err = "non-finite state: " + std::to_string(nbad) + " instance(s) stopped, first: " + which;
+    return ZK_ERR_NUMERIC;
 }
 
 zk_status zk_ensemble_size(zk_ensemble *E, int *ninst, int64_t *updated_cells)

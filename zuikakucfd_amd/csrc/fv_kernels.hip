@@ -74,8 +74,7 @@ __global__ __launch_bounds__(256) void fv_max_speed_kernel(const double *__restr
         const double u = U[np + k] / rho, v = U[2 * np + k] / rho;
         const double p = (gamma - 1.0) * (U[3 * np + k] - 0.5 * rho * (u * u + v * v));
         const double a = sqrt(gamma * p / rho);
-        const double spd = sqrt(u * u + v * v) + a;
-        if (spd > m) m = spd;
+        m = speed_max(m, sqrt(u * u + v * v) + a, finite4(rho, U[np + k], U[2 * np + k], U[3 * np + k]));
     }
     block_max_to(m, out);
 }

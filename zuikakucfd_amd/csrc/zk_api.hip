@@ -26,6 +26,16 @@ namespace {
 
 thread_local std::string g_create_err;
 
+}  // namespace
+
+namespace zk {
+// The message zk_last_error(NULL) returns: that of the calling thread's last
+// failed call made without a handle (zk_create and the like).
+std::string &null_handle_error() { return g_create_err; }
+}  // namespace zk
+
+namespace {
+
 #define ZK_HIP(call)                                                                      \
     do {                                                                                  \
         hipError_t _e = (call);                                                           \
@@ -53,11 +63,19 @@ __global__ void advance_time_kernel(zk::StepScalars *S)
 
 // dt = cfl * da / lambda, clipped so that t + dt <= t_end; marks done once t
 // has reached t_end so that the stage kernels of later steps become no-ops.
+// A wave-speed maximum that is not finite (a broken state) stops the stepping
+// the same way and sets bad.
 __global__ void cfl_dt_kernel(zk::StepScalars *S, const unsigned long long *lam_bits, double da)
 {
     if (threadIdx.x != 0) return;
     const double lam = __longlong_as_double((long long)*lam_bits);
     S->lambda = lam;
+    if (!(lam <= 1.79769313486231570815e308)) {
+        S->bad = 1;
+        S->done = 1;
+        S->dt = 0.0;
+        return;
+    }
     if (S->t >= S->t_end) {
         S->done = 1;
         S->dt = 0.0;
@@ -66,6 +84,58 @@ __global__ void cfl_dt_kernel(zk::StepScalars *S, const unsigned long long *lam_
     double dt = S->cfl * da / lam;
     if (S->t + dt > S->t_end) dt = S->t_end - S->t;
     S->dt = dt;
+}
+
+// Non-finite check of a state's means (mode 0 of the four variables, every
+// cell): records the smallest cell*4 + variable found and the step count
+// `step` in health[0], health[1] (atomic minima; the record starts at all
+// ones).  Reads 32 bytes per cell, a fraction of one stage's traffic.
+__global__ __launch_bounds__(256) void health_kernel(const double *__restrict__ Q, size_t np, int nm,
+                                                     unsigned long long *health, unsigned long long step)
+{
+    unsigned long long key = ~0ull;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (size_t)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double x = Q[(size_t)c * nm * np + k];
+            if (!(fabs(x) <= 1.79769313486231570815e308)) {
+                const unsigned long long kk = (unsigned long long)k * 4 + c;
+                if (kk < key) key = kk;
+            }
+        }
+    if (key != ~0ull) {
+        atomicMin(&health[0], key);
+        atomicMin(&health[1], step);
+    }
+}
+
+void enqueue_health(Handle *H)
+{
+    int blocks = (int)((H->g.np + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(health_kernel, dim3(blocks), dim3(256), 0, H->stream, H->q, H->g.np, H->nm, H->health,
+                       (unsigned long long)H->steps_enqueued);
+}
+
+// Reads the non-finite record after the handle's stream has drained.  A
+// non-finite state makes this and every later stepping call fail with
+// ZK_ERR_NUMERIC (until zk_load or zk_set_state replaces the state).
+zk_status check_health(Handle *H)
+{
+    unsigned long long v[2] = {~0ull, ~0ull};
+    ZK_HIP(hipMemcpyAsync(v, H->health, sizeof(v), hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    if (v[0] == ~0ull) return ZK_OK;
+    const unsigned long long cell = v[0] / 4;
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "non-finite state: U%d (conserved variable %d) of cell (i=%llu, j=%llu) is not finite, found by "
+                  "the check after step %llu",
+                  (int)(v[0] % 4), (int)(v[0] % 4), cell / (unsigned long long)H->g.nx,
+                  cell % (unsigned long long)H->g.nx, v[1]);
+    H->err = msg;
+    H->numeric_failed = true;
+    return ZK_ERR_NUMERIC;
 }
 
 zk_status check_args(Handle *H)
@@ -92,13 +162,14 @@ void free_all(Handle *H)
 {
     drop_graph(H);
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->srcy, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
-                    H->scal, H->lam_bits, H->bct, H->stats, H->score, H->cand, H->bad, H->any, H->blist, H->zsok, H->tlist};
+                    H->scal, H->lam_bits, H->health, H->bct, H->stats, H->score, H->cand, H->bad, H->any, H->blist,
+                    H->zsok, H->tlist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
     H->src = nullptr; H->srcy = nullptr; H->upd = nullptr; H->ring = nullptr;
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
-    H->scal = nullptr; H->lam_bits = nullptr; H->bct = nullptr;
+    H->scal = nullptr; H->lam_bits = nullptr; H->health = nullptr; H->bct = nullptr;
     H->stats = nullptr; H->score = nullptr; H->cand = nullptr; H->bad = nullptr; H->any = nullptr; H->blist = nullptr; H->zsok = nullptr; H->tlist = nullptr;
     H->loaded = false;
 }
@@ -219,6 +290,39 @@ zk_status enqueue_step(Handle *H)
     if (e != hipSuccess) {
         H->err = std::string("kernel launch: ") + hipGetErrorString(e);
         return ZK_ERR_HIP;
+    }
+    return ZK_OK;
+}
+
+// Enqueues nsteps steps with the dt in device memory: one captured step
+// replayed, or the launches themselves.
+zk_status enqueue_steps(Handle *H, int nsteps)
+{
+    zk_status st = ZK_OK;
+    // One step (dt is read from device memory) is captured once and replayed,
+    // which removes the per-launch gaps (a DG step is about 15 launches).  Not
+    // for the learned limiter or harvesting, which salt their hashes per stage.
+    const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
+    if (!salted && H->use_graph && !H->prof && nsteps >= 2) {
+        if (!H->step_exec) {
+            ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
+            st = enqueue_step(H);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(H->stream, &g);
+            if (st) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            ZK_HIP(e);
+            H->step_graph = g;
+            ZK_HIP(hipGraphInstantiate(&H->step_exec, g, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; ++s) ZK_HIP(hipGraphLaunch(H->step_exec, H->stream));
+        return ZK_OK;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        st = enqueue_step(H);
+        if (st) return st;
     }
     return ZK_OK;
 }
@@ -398,6 +502,10 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     ZK_HIP(hipMalloc(&H->flags, np));
     ZK_HIP(hipMalloc(&H->scal, sizeof(zk::StepScalars)));
     ZK_HIP(hipMalloc(&H->lam_bits, sizeof(unsigned long long)));
+    ZK_HIP(hipMalloc(&H->health, 2 * sizeof(unsigned long long)));
+    ZK_HIP(hipMemset(H->health, 0xff, 2 * sizeof(unsigned long long)));
+    H->steps_enqueued = 0;
+    H->numeric_failed = false;
     if (H->cfg.method == ZK_METHOD_WENO && H->cfg.positivity) {
         ZK_HIP(hipMalloc(&H->bad, np));
         ZK_HIP(hipMemset(H->bad, 0, np));
@@ -445,6 +553,7 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     ZK_HIP(hipMemset(H->lam_bits, 0, sizeof(unsigned long long)));
     ZK_HIP(hipDeviceSynchronize());
     H->loaded = true;
+    enqueue_health(H);  // an input that is not finite is reported by the next synchronising call
     return ZK_OK;
 }
 
@@ -460,8 +569,11 @@ zk_status zk_set_state(zk_handle *h, const double *Q)
     ZK_HIP(hipSetDevice(H->cfg.device));
     ZK_HIP(hipMemcpyAsync(H->q, Q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyHostToDevice,
                           H->stream));
-    ZK_HIP(hipStreamSynchronize(H->stream));
-    return ZK_OK;
+    // a new state starts a new record
+    ZK_HIP(hipMemsetAsync(H->health, 0xff, 2 * sizeof(unsigned long long), H->stream));
+    H->numeric_failed = false;
+    enqueue_health(H);
+    return check_health(H);  // the state is set either way
 }
 
 zk_status zk_get_state(zk_handle *h, double *Q)
@@ -477,7 +589,7 @@ zk_status zk_get_state(zk_handle *h, double *Q)
     ZK_HIP(hipMemcpyAsync(Q, H->q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyDeviceToHost,
                           H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
-    return ZK_OK;
+    return check_health(H);
 }
 
 zk_status zk_get_conserved(zk_handle *h, double *U0, double *U1, double *U2, double *U3)
@@ -497,7 +609,7 @@ zk_status zk_get_conserved(zk_handle *h, double *U0, double *U1, double *U2, dou
                               hipMemcpyDeviceToHost, H->stream));
     }
     ZK_HIP(hipStreamSynchronize(H->stream));
-    return ZK_OK;
+    return check_health(H);
 }
 
 zk_status zk_max_speed(zk_handle *h, double *out)
@@ -535,33 +647,13 @@ zk_status zk_step(zk_handle *h, double dt, int nsteps)
         H->err = "zk_step: need dt >= 0 and nsteps >= 0";
         return ZK_ERR_ARG;
     }
+    if (H->numeric_failed) return ZK_ERR_NUMERIC;  // H->err still names the cell
     ZK_HIP(hipSetDevice(H->cfg.device));
     hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, dt);
-    // One step (dt is read from device memory) is captured once and replayed,
-    // which removes the per-launch gaps (a DG step is about 15 launches).  Not
-    // for the learned limiter or harvesting, which salt their hashes per stage.
-    const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
-    if (!salted && H->use_graph && !H->prof && nsteps >= 2) {
-        if (!H->step_exec) {
-            ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
-            st = enqueue_step(H);
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(H->stream, &g);
-            if (st) {
-                if (g) (void)hipGraphDestroy(g);
-                return st;
-            }
-            ZK_HIP(e);
-            H->step_graph = g;
-            ZK_HIP(hipGraphInstantiate(&H->step_exec, g, nullptr, nullptr, 0));
-        }
-        for (int s = 0; s < nsteps; ++s) ZK_HIP(hipGraphLaunch(H->step_exec, H->stream));
-        return ZK_OK;
-    }
-    for (int s = 0; s < nsteps; ++s) {
-        st = enqueue_step(H);
-        if (st) return st;
-    }
+    st = enqueue_steps(H, nsteps);
+    if (st) return st;
+    H->steps_enqueued += nsteps;
+    if (nsteps > 0) enqueue_health(H);
     return ZK_OK;
 }
 
@@ -574,6 +666,19 @@ zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps, double
         H->err = "zk_step_cfl: need cfl > 0 and nsteps >= 0";
         return ZK_ERR_ARG;
     }
+    if (H->cfg.method == ZK_METHOD_DG) {
+        // the stable limit of SSP-RK3 DG P, 0.9 / (2 (2P + 1)); the reference
+        // refuses to run above it (src_DG/solver.cpp:12-17, 245-260)
+        const double lim = 0.9 / (2.0 * (2 * H->cfg.P + 1));
+        if (cfl > lim) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "zk_step_cfl: CFL %.6g exceeds the DG P=%d stable limit %.6g", cfl,
+                          H->cfg.P, lim);
+            H->err = msg;
+            return ZK_ERR_CFL;
+        }
+    }
+    if (H->numeric_failed) return ZK_ERR_NUMERIC;  // H->err still names the cell
     ZK_HIP(hipSetDevice(H->cfg.device));
     zk::StepScalars s{};
     ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
@@ -598,11 +703,28 @@ zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps, double
         st = enqueue_step(H);
         if (st) return st;
         hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, H->stream, H->scal);
+        ++H->steps_enqueued;
     }
     ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
     if (t_out) *t_out = s.t;
     if (steps_out) *steps_out = s.steps - steps0;
+    enqueue_health(H);
+    st = check_health(H);
+    if (st) return st;
+    if (s.bad) {
+        // finite conserved values whose wave speed is not (zero density): the
+        // stepping stopped at the step that would have divided by it
+        const int at = s.steps - steps0;
+        s.bad = 0;
+        ZK_HIP(hipMemcpyAsync(H->scal, &s, sizeof(s), hipMemcpyHostToDevice, H->stream));
+        ZK_HIP(hipStreamSynchronize(H->stream));
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "zk_step_cfl: the wave-speed maximum is not finite after %d steps "
+                      "(a cell with zero density)", at);
+        H->err = msg;
+        return ZK_ERR_NUMERIC;
+    }
     return ZK_OK;
 }
 
@@ -612,7 +734,7 @@ zk_status zk_sync(zk_handle *h)
     if (!H) return ZK_ERR_ARG;
     ZK_HIP(hipSetDevice(H->cfg.device));
     ZK_HIP(hipStreamSynchronize(H->stream));
-    return ZK_OK;
+    return H->loaded ? check_health(H) : ZK_OK;
 }
 
 zk_status zk_step_timed(zk_handle *h, double dt, int nsteps, float *ms)
@@ -638,7 +760,7 @@ zk_status zk_step_timed(zk_handle *h, double dt, int nsteps, float *ms)
     if (ms) *ms = t;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return ZK_OK;
+    return check_health(H);
 }
 
 zk_status zk_get_flags(zk_handle *h, uint8_t *flags)

@@ -142,8 +142,25 @@ __device__ __forceinline__ void hll_f64(const double L[4], const double R[4], do
     }
 }
 
+// Running maximum of wave speeds with the reference's comparison (a NaN
+// speed, from a negative pressure of a finite state, never wins; an infinite
+// one, from a zero density, does), except that a state whose conserved values
+// are not all finite gives +inf, so the wave-speed reduction reports a broken
+// state instead of skipping it.
+__device__ __forceinline__ bool finite4(double a, double b, double c, double d)
+{
+    constexpr double M = 1.79769313486231570815e308;
+    return fabs(a) <= M && fabs(b) <= M && fabs(c) <= M && fabs(d) <= M;
+}
+__device__ __forceinline__ double speed_max(double m, double s, bool finite_state)
+{
+    if (!finite_state) return __builtin_huge_val();
+    return s > m ? s : m;
+}
+
 // Block-wide max of a non-negative double, merged into *out with one
-// atomicMax on the IEEE bit pattern (order-preserving for values >= 0).
+// atomicMax on the IEEE bit pattern (order-preserving for values >= 0; +inf,
+// which speed_max gives a non-finite state, orders above every finite value).
 // NaN never wins: the caller only raises m on a true comparison.
 __device__ __forceinline__ void block_max_to(double m, unsigned long long *out)
 {

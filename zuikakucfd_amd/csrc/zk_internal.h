@@ -31,6 +31,8 @@ struct StepScalars {
     double cfl;       // CFL number for CFL stepping
     int32_t done;     // 1 once t >= t_end (CFL stepping stops updating)
     int32_t steps;    // steps taken by CFL stepping
+    int32_t bad;      // 1 once a wave-speed maximum was not finite (CFL stepping stops)
+    int32_t pad;
 };
 
 struct Grid {
@@ -76,6 +78,12 @@ struct Handle {
     uint8_t *flags = nullptr; // limiter flags
     StepScalars *scal = nullptr;          // device
     unsigned long long *lam_bits = nullptr; // device max-reduce target
+    // non-finite state check, run after every zk_step / zk_step_cfl / zk_set_state:
+    // [0] = smallest cell*4 + variable of a non-finite mean, [1] = smallest step
+    // count at which a check saw one; both all ones while the state is finite
+    unsigned long long *health = nullptr;
+    long long steps_enqueued = 0;   // steps enqueued since zk_load (the count the checks record)
+    bool numeric_failed = false;    // a synchronising call found a non-finite state
     int32_t *bct = nullptr;   // bctype copy (device)
     std::vector<int32_t> h_bct;
 

@@ -56,7 +56,25 @@ class ZkConfig(C.Structure):
 
 
 class ZkError(RuntimeError):
-    pass
+    """A failed zk_* call; `status` holds the ZK_ERR_* code (0 when none)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
+
+
+class ZkNumericError(ZkError):
+    """ZK_ERR_NUMERIC: the state is no longer finite (the message names the
+    variable, the cell and the step count of the check that found it)."""
+
+
+class ZkCflError(ZkError):
+    """ZK_ERR_CFL: a DG CFL number above the stable limit 0.9 / (2 (2P + 1))."""
+
+
+def _raise(st: int, msg: str):
+    cls = ZkNumericError if st == 3 else ZkCflError if st == 4 else ZkError
+    raise cls(msg, st)
 
 
 _lib = None
@@ -179,7 +197,7 @@ class Solver:
     def _check(self, st: int, what: str) -> None:
         if st != ZK_OK:
             msg = self.lib.zk_last_error(self.h).decode()
-            raise ZkError(f"{what}: {_ERR_NAMES.get(st, st)}: {msg}")
+            _raise(st, f"{what}: {_ERR_NAMES.get(st, st)}: {msg}")
 
     # -- data -----------------------------------------------------------
     def load(self, da: float, gamma: float, rho, u, v, p, bctype) -> None:
@@ -310,7 +328,7 @@ class Ensemble:
 
     def _check(self, st: int, what: str) -> None:
         if st != ZK_OK:
-            raise ZkError(f"{what}: {_ERR_NAMES.get(st, st)}: {self.lib.zk_ensemble_last_error(self.h).decode()}")
+            _raise(st, f"{what}: {_ERR_NAMES.get(st, st)}: {self.lib.zk_ensemble_last_error(self.h).decode()}")
 
     def add(self, da, gamma, rho, u, v, p, bctype, cfl: float) -> int:
         rho = np.ascontiguousarray(rho, dtype=np.float64)
