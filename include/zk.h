@@ -264,6 +264,30 @@ zk_status zk_learned_stats(zk_handle *h, int64_t out[8], int reset);
  * current state (cells as row-major indices inside [3,ny-3)x[3,nx-3)). */
 zk_status zk_mlp_features(zk_handle *h, zk_mlp *m, const int32_t *cells, int n, uint16_t *X, float *scores);
 
+/* ---------------------------------------------------------------------------
+ * WENO stepping that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It advances the caller's
+ * device state Q (4 row-major ny*nx planes of conserved variables rho, rho*u,
+ * rho*v, E in device memory on cfg->device) by nsteps fixed-dt SSP-RK3 steps
+ * in place, reading the bctype codes from device memory (0 updated, -1 copy
+ * ghost, -2 periodic ghost; other codes are never filled), and enqueues on
+ * `stream` (a hipStream_t, or NULL).  Results equal zk_step on a WENO handle
+ * loaded with the same state, bit for bit.  cfg: ZK_METHOD_WENO, FP64 or FP32
+ * bulk, HLLC or HLL, positivity 0; the reconstruction is WENO5.
+ *
+ * Its stage buffers, ghost map, ring list and step scalars live in one device
+ * array of ZK_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), so
+ * grids are limited to ZK_NOALLOC_MAX_CELLS cells (ny*nx); larger ones return
+ * ZK_ERR_ARG.  Calls on one GPU share that array: calls that overlap on one
+ * GPU (different streams or threads) race.  Errors: zk_last_error(NULL).  The
+ * first call on each device looks up the array's address; make it outside
+ * stream capture.  The call is asynchronous and does no failure check. */
+#define ZK_NOALLOC_MAX_CELLS (1u << 20)
+#define ZK_NOALLOC_SCRATCH_BYTES 76546304u
+zk_status zk_weno_step_noalloc(const zk_config *cfg, int ny, int nx, double da, double gamma,
+                               double *Q, const int32_t *bctype, double dt, int nsteps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

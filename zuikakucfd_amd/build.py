@@ -5,8 +5,9 @@
 * ``bin/zk_weno``, ``bin/zk_dg{2,3,4}`` -- the drop-in host executables
                     (C++17, g++) that read params.txt/rho,u,v,p,bctype.dat and
                     drive libzk.so with the reference's command line.
-* ``tests/_bin/*_check`` -- device-arithmetic check programs the GPU tests
-                    run (from ``tests/*_check.hip``).
+* ``tests/_bin/*_check`` -- check programs the tests run: device arithmetic
+                    (``tests/*_check.hip``) and host checks of the library's
+                    headers against the oracle (``tests/*_check.cpp``).
 * the oracle     -- ``oracle/liboracle.so`` via ``make -C oracle`` (test
                     infrastructure; built here so a checkout has one entry).
 
@@ -41,7 +42,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}",
 ]
-HIP_SOURCES = ["zk_api.hip", "fv_kernels.hip", "dg_kernels.hip", "ens_kernels.hip", "mlp_kernels.hip"]
+HIP_SOURCES = ["zk_api.hip", "fv_kernels.hip", "dg_kernels.hip", "ens_kernels.hip", "mlp_kernels.hip",
+               "noalloc.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
@@ -91,13 +93,20 @@ def build_bins() -> list[Path]:
 
 
 def build_checks() -> list[Path]:
-    """Device-arithmetic check programs that the GPU tests run (tests/_bin/)."""
+    """Check programs the tests run (tests/_bin/): device-arithmetic checks
+    (tests/*_check.hip, hipcc) and host checks of the library's headers against
+    the oracle (tests/*_check.cpp, g++, linked with oracle/liboracle.so)."""
     out_dir = ROOT / "tests" / "_bin"
     out_dir.mkdir(exist_ok=True)
     outs = []
     for src in sorted((ROOT / "tests").glob("*_check.hip")):
         out = out_dir / src.stem
         _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", str(src), "-o", str(out)])
+        outs.append(out)
+    for src in sorted((ROOT / "tests").glob("*_check.cpp")):
+        out = out_dir / src.stem
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{CSRC}", f"-I{ROOT / 'oracle'}", str(src),
+              f"-L{ROOT / 'oracle'}", "-loracle", "-Wl,-rpath,$ORIGIN/../../oracle", "-o", str(out)])
         outs.append(out)
     return outs
 
@@ -111,8 +120,8 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_lib()
     build_bins()
-    build_checks()
     build_oracle()
+    build_checks()
 
 
 if __name__ == "__main__":

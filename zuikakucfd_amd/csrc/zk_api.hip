@@ -30,7 +30,7 @@ thread_local std::string g_create_err;
 
 namespace zk {
 // The message zk_last_error(NULL) returns: that of the calling thread's last
-// failed call made without a handle (zk_create and the like).
+// failed call made without a handle (zk_create, zk_weno_step_noalloc).
 std::string &null_handle_error() { return g_create_err; }
 }  // namespace zk
 

@@ -38,6 +38,7 @@ EXPORTED = [
     "zk_mlp_create", "zk_mlp_destroy", "zk_mlp_last_error", "zk_mlp_param_count", "zk_mlp_get_params",
     "zk_mlp_set_params", "zk_mlp_grad", "zk_mlp_infer", "zk_mlp_dataset", "zk_mlp_clear_dataset", "zk_mlp_train",
     "zk_harvest", "zk_attach_mlp", "zk_learned_stats", "zk_mlp_features",
+    "zk_weno_step_noalloc",
 ]
 _NO_STATUS = ("zk_last_error", "zk_ensemble_last_error", "zk_mlp_last_error", "zk_mlp_param_count")
 
@@ -142,6 +143,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_attach_mlp.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float]
     lib.zk_learned_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
     lib.zk_mlp_features.argtypes = [vp, vp, C.POINTER(C.c_int32), C.c_int, u16p, fp]
+    lib.zk_weno_step_noalloc.argtypes = [C.POINTER(ZkConfig), C.c_int, C.c_int, C.c_double, C.c_double, vp, vp,
+                                         C.c_double, C.c_int, vp]
     for name in EXPORTED:
         if name not in _NO_STATUS:
             getattr(lib, name).restype = C.c_int
@@ -374,6 +377,21 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def weno_step_noalloc(Q_ptr: int, bctype_ptr: int, ny: int, nx: int, da: float, gamma: float, dt: float,
+                      nsteps: int, stream: int = 0, precision: int = PREC_FP64, flux: int = FLUX_HLLC,
+                      device: int = 0) -> None:
+    """zk_weno_step_noalloc on device buffers given by address: Q_ptr (4
+    row-major ny*nx float64 planes, advanced in place) and bctype_ptr (ny*nx
+    int32), enqueued on `stream` (a hipStream_t address, 0 for the default
+    stream).  The library allocates nothing."""
+    lib = load_library()
+    cfg = ZkConfig(METHOD_WENO, 0, precision, flux, 0, 0.0, device, 0)
+    st = lib.zk_weno_step_noalloc(C.byref(cfg), int(ny), int(nx), float(da), float(gamma), C.c_void_p(Q_ptr),
+                                  C.c_void_p(bctype_ptr), float(dt), int(nsteps), C.c_void_p(stream or None))
+    if st != ZK_OK:
+        _raise(st, f"zk_weno_step_noalloc: {_ERR_NAMES.get(st, st)}: {lib.zk_last_error(None).decode()}")
 
 
 def _f32p(a):
