@@ -1,0 +1,272 @@
+"""Pins of oracle branches that the end-to-end tests leave open.
+
+Each case has an expected value fixed by a closed form or a textbook
+identity, chosen so that a plausible mistake in the oracle (a quadratic
+instead of linear positivity theta, the full Legendre basis where the
+reference truncates it, the P < 3 Persson threshold at P = 3, the floored
+instead of the raw conserved density in the HLLC jump term, a wrong star
+state) fails one of them.  References: applyLimiter, src_DG/fluid.cpp:824-992
+(basis() :808-823); hllcFlux (DG), src_DG/fluid.cpp:450-520.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref
+import oracle
+from zuikakucfd_amd import meshio
+
+G = 1.4
+GP4 = (-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526)
+
+
+def leg(p, x):
+    return np.polynomial.legendre.legval(x, [0.0] * p + [1.0])
+
+
+def cell_state(P, n=8):
+    nm = (P + 1) ** 2
+    Q = np.zeros((4, nm, n, n))
+    Q[0, 0] = 1.0
+    Q[3, 0] = 2.5
+    return Q, meshio.ring_bctype(n, n)
+
+
+# --------------------------------------------------------------------------
+# Zhang-Shu positivity scaling (src_DG/fluid.cpp:892-991)
+# --------------------------------------------------------------------------
+def test_zs_pressure_theta_is_linear():
+    """Mean state rho = 1, E = 1, no mean momentum; an x-linear momentum mode
+    m1 = 2 makes p = (g-1)(1 - (m1 xi)^2 / 2) negative at xi = +-0.8611.
+    Every mode >= 1 is scaled by the linear theta
+    (p_avg - 1e-5) / (p_avg - p_min + 1e-5), p_avg = (g-1) E, not by the root
+    of p(theta) = 1e-5 that Zhang-Shu's quadratic form would take."""
+    Q, bct = cell_state(1)
+    Q[3, 0, 4, 4] = 1.0
+    Q[1, 2, 4, 4] = 2.0            # mode m = px*(P+1)+py = 2: (px, py) = (1, 0)
+    o = oracle.DG(1, 0.1, G, Q, bct)
+    o.limit_zs()
+    p_avg = (G - 1) * 1.0
+    p_min = (G - 1) * (1.0 - 0.5 * (2.0 * GP4[3]) ** 2)
+    theta = (p_avg - 1e-5) / (p_avg - p_min + 1e-5)
+    assert o.Q[1, 2, 4, 4] == pytest.approx(2.0 * theta, rel=1e-14)
+    assert o.Q[3, 0, 4, 4] == 1.0 and o.Q[0, 0, 4, 4] == 1.0
+    # the quadratic solve gives a different factor
+    theta_q = np.sqrt((1.0 - 1e-5 / (G - 1)) / (0.5 * (2.0 * GP4[3]) ** 2))
+    assert abs(theta_q - theta) > 1e-3
+
+
+def test_zs_pressure_reset_of_the_mean():
+    """A mean pressure below 1e-5 resets the mean energy to
+    1e-5/(g-1) + |m|^2/(2 rho) (not conservative), before any scaling."""
+    Q, bct = cell_state(2)
+    Q[0, 0, 4, 4], Q[1, 0, 4, 4], Q[3, 0, 4, 4] = 1.0, 1.0, 0.5   # p_avg = 0
+    o = oracle.DG(2, 0.1, G, Q, bct)
+    o.limit_zs()
+    assert o.Q[3, 0, 4, 4] == pytest.approx(1e-5 / (G - 1) + 0.5, rel=1e-15)
+    assert o.Q[1, 0, 4, 4] == 1.0
+
+
+def test_zs_density_floor_of_the_mean_on_every_cell():
+    """A negative mean density becomes 1e-5 (mass is not conserved), also in
+    a cell inside [3, n-3) whose bctype is not 0: the scaling has no bctype
+    test."""
+    Q, bct = cell_state(1, n=10)
+    Q[0, 0, 4, 4] = -0.3
+    Q[0, 0, 5, 5] = -0.7
+    bct[5, 5] = -1
+    o = oracle.DG(1, 0.1, G, Q, bct)
+    o.limit_zs()
+    assert o.Q[0, 0, 4, 4] == 1e-5
+    assert o.Q[0, 0, 5, 5] == 1e-5
+    # outside [3, n-3) nothing changes
+    Q2, _ = cell_state(1, n=10)
+    Q2[0, 0, 1, 1] = -0.3
+    o2 = oracle.DG(1, 0.1, G, Q2, bct)
+    o2.limit_zs()
+    assert o2.Q[0, 0, 1, 1] == -0.3
+
+
+def test_zs_ignores_cubic_modes_at_p3():
+    """basis() defines Legendre P0..P2 only, so at P = 3 a cubic mode never
+    enters the positivity test: a density (3,0) mode of 3 makes the full
+    polynomial negative at xi = 0.34, yet the cell is left alone."""
+    Q, bct = cell_state(3)
+    Q[0, 12, 4, 4] = 3.0           # (px, py) = (3, 0)
+    full_min = min(1.0 + 3.0 * leg(3, x) for x in GP4)
+    assert full_min < 0.0
+    o = oracle.DG(3, 0.1, G, Q, bct)
+    o.limit_zs()
+    np.testing.assert_array_equal(o.Q, Q)
+
+
+def test_zs_theta_from_the_truncated_polynomial_at_p3():
+    """With a quadratic (2,0) mode of -2 and a cubic (3,0) mode of 0.5, theta
+    comes from 1 - 2 P2(xi) alone; every mode >= 1 (the cubic included) is
+    scaled by it."""
+    Q, bct = cell_state(3)
+    Q[0, 8, 4, 4] = -2.0           # (2, 0)
+    Q[0, 12, 4, 4] = 0.5           # (3, 0)
+    o = oracle.DG(3, 0.1, G, Q, bct)
+    o.limit_zs()
+    rmin = min(1.0 - 2.0 * leg(2, x) for x in GP4)
+    theta = (1.0 - 1e-5) / (1.0 - rmin + 1e-5)
+    full = min(1.0 - 2.0 * leg(2, x) + 0.5 * leg(3, x) for x in GP4)
+    assert abs((1.0 - 1e-5) / (1.0 - full + 1e-5) - theta) > 1e-3
+    assert o.Q[0, 8, 4, 4] == pytest.approx(-2.0 * theta, rel=1e-14)
+    assert o.Q[0, 12, 4, 4] == pytest.approx(0.5 * theta, rel=1e-14)
+
+
+# --------------------------------------------------------------------------
+# Persson filter (src_DG/fluid.cpp:824-887)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("delta", [-0.01, 0.01])
+def test_persson_threshold_at_p3(delta):
+    """At P = 3 the threshold is s0 = -3 log10 3 (not -4 log10 P as below
+    P = 3).  A (3,0) density mode sized so that s = s0 + delta: just below,
+    untouched; just above, scaled by 1 - eps with
+    eps = (1 + sin(pi (delta - 1/2))) / 2."""
+    s0 = -3.0 * np.log10(3.0)
+    R = 10.0 ** (s0 + delta)
+    a = np.sqrt(R * (1.0 + 1e-5) / (1.0 - R))   # a^2 / (1 + a^2 + 1e-5) = R
+    Q, bct = cell_state(3)
+    Q[0, 12, 4, 4] = a
+    o = oracle.DG(3, 0.1, G, Q, bct, limiter=1)
+    o.limit_persson_zs()
+    if delta < 0:
+        assert o.Q[0, 12, 4, 4] == a
+    else:
+        eps = 0.5 * (1.0 + np.sin(np.pi * (delta - 0.5)))
+        assert 0.0 < eps < 1e-3
+        assert o.Q[0, 12, 4, 4] == pytest.approx(a * (1.0 - eps), rel=1e-9)
+    # under the P < 3 rule (-4 log10 3) both cases would be limited hard
+    assert s0 + delta > -4.0 * np.log10(3.0) + 0.4
+
+
+# --------------------------------------------------------------------------
+# HLLC of the DG path (src_DG/fluid.cpp:450-520)
+# --------------------------------------------------------------------------
+def cons(r, u, v, p, g=G):
+    return np.array([r, r * u, r * v, p / (g - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def phys(U, g=G):
+    r, u, v = U[0], U[1] / U[0], U[2] / U[0]
+    p = (g - 1) * (U[3] - 0.5 * r * (u * u + v * v))
+    return np.array([r * u, r * u * u + p, r * u * v, (U[3] + p) * u]), p
+
+
+def toro_star(U, S, Ss, g=G):
+    """Star state of Toro, Riemann Solvers, 3rd ed., eq. (10.39)."""
+    r, u, v = U[0], U[1] / U[0], U[2] / U[0]
+    _, p = phys(U, g)
+    f = r * (S - u) / (S - Ss)
+    return f * np.array([1.0, Ss, v, U[3] / r + (Ss - u) * (Ss + p / (r * (S - u)))])
+
+
+def test_hllc_dg_symmetric_form_identity():
+    """Away from the floors, the branch form of HLLC equals Toro's symmetric
+    form (eq. 10.71): F = (FL + FR)/2 - (|SL|(UL* - UL) + |S*|(UR* - UL*) +
+    |SR|(UR - UR*))/2, with both star states of eq. (10.39).  The identity
+    holds only if the star states satisfy the contact conditions, so it pins
+    the star-state arithmetic of both sides and the branch choice."""
+    rng = np.random.default_rng(5)
+    n = 0
+    while n < 400:
+        rl, rr = rng.uniform(0.05, 3.0, 2)
+        pl, pr = rng.uniform(0.05, 3.0, 2)
+        ul, ur, vl, vr = rng.uniform(-2.0, 2.0, 4)
+        L, R = cons(rl, ul, vl, pl), cons(rr, ur, vr, pr)
+        SL, SR, Ss = oracle.hllc_speeds(1, L, R, G)
+        if not (SL < 0.0 < SR):
+            continue
+        FL, _ = phys(L)
+        FR, _ = phys(R)
+        UsL, UsR = toro_star(L, SL, Ss), toro_star(R, SR, Ss)
+        F_sym = 0.5 * (FL + FR) - 0.5 * (abs(SL) * (UsL - L) + abs(Ss) * (UsR - UsL) + abs(SR) * (R - UsR))
+        F = oracle.flux("hllc_dg", L, R, G)
+        np.testing.assert_allclose(F, F_sym, rtol=1e-11, atol=1e-11)
+        n += 1
+
+
+def test_hllc_dg_jump_uses_the_raw_density():
+    """The density is floored at 1e-5 for the primitive variables, but the jump
+    term S (U* - U) takes the conserved density as given: a left state with
+    density -1 gives the flux of density 1e-5 plus SL (1 + 1e-5) in the mass
+    component, and the same other components (the left star branch: the
+    higher left pressure gives S* > 0)."""
+    R = cons(1.0, 0.0, 0.0, 1.0)
+    L_raw = np.array([-1.0, 0.0, 0.0, 5.0])
+    L_flo = np.array([1e-5, 0.0, 0.0, 5.0])
+    SL, _, Ss = oracle.hllc_speeds(1, L_raw, R, G)
+    assert Ss > 0.0 and SL < 0.0
+    F_raw = oracle.flux("hllc_dg", L_raw, R, G)
+    F_flo = oracle.flux("hllc_dg", L_flo, R, G)
+    assert F_raw[0] == pytest.approx(F_flo[0] + SL * (1.0 + 1e-5), rel=1e-12)
+    np.testing.assert_array_equal(F_raw[1:], F_flo[1:])
+
+
+def test_hllc_dg_guards_that_cannot_bind():
+    """For finite states the Roe-denominator floor (sqrt(rho_L) + sqrt(rho_R)
+    >= 2 sqrt(1e-5) > 1e-5) and the S* denominator floor never bind:
+    |rho_L (S_L - u_L) - rho_R (S_R - u_R)| >= rho_L a_L + rho_R a_R
+    >= 2 sqrt(1e-5 * max(g 1e-5, 1e-5 * 1e-5)) > 1e-5 with the density,
+    pressure and sound-speed floors.  Brute force over states down to and
+    below the floors: the bound holds and every flux is finite."""
+    rng = np.random.default_rng(9)
+    worst = np.inf
+    for _ in range(20000):
+        rl, rr = 10.0 ** rng.uniform(-7, 3, 2)
+        pl, pr = 10.0 ** rng.uniform(-7, 3, 2)
+        ul, ur, vl, vr = rng.uniform(-50.0, 50.0, 4)
+        L, R = cons(rl, ul, vl, pl), cons(rr, ur, vr, pr)
+        SL, SR, _ = oracle.hllc_speeds(1, L, R, G)
+        rL, rR = max(rl, 1e-5), max(rr, 1e-5)
+        den2 = rL * (SL - L[1] / rL) - rR * (SR - R[1] / rR)
+        worst = min(worst, abs(den2))
+        assert np.isfinite(oracle.flux("hllc_dg", L, R, G)).all()
+    assert worst > 2e-5
+
+
+# --------------------------------------------------------------------------
+# Features of the learned indicator (oracle/mlp_ref.py)
+# --------------------------------------------------------------------------
+def p2_state(n=9):
+    return np.zeros((4, 9, n, n))
+
+
+def test_features_constant_field_is_zero():
+    Q = p2_state()
+    Q[:, 0] = np.array([1.3, 0.4, -0.2, 3.0])[:, None, None]
+    X = mlp_ref.features(Q, [(4, 4), (3, 5)])
+    assert not X.any()
+
+
+def test_features_scale_invariant():
+    """Every variable scaled by k > 0 leaves the features unchanged (up to the
+    1e-12 in the scale and the FP32 rounding)."""
+    rng = np.random.default_rng(2)
+    Q = rng.normal(size=(4, 9, 9, 9))
+    Q[0, 0] += 3.0
+    X1 = mlp_ref.features(Q, [(4, 4)])
+    X2 = mlp_ref.features(3.7 * Q, [(4, 4)])
+    np.testing.assert_allclose(X2, X1, rtol=2e-7, atol=1e-7)
+
+
+def test_features_slot_order():
+    """A mean varying along j only, U = 1 + b j, puts +b / s in the E slot
+    (n = 1, j+1), -b / s in the W slot (n = 2, j-1) and nothing in the self,
+    S (i+1) and N (i-1) slots; a mode enters at n*36 + c*9 + m."""
+    Q = p2_state()
+    b = 0.125
+    jj = np.arange(9)[None, :] * np.ones((9, 1))
+    Q[0, 0] = 2.0                  # density: constant
+    Q[3, 0] = 1.0 + b * jj         # energy: ramp along j
+    Q[1, 4, 4, 4] = 0.5            # one higher mode of the cell itself: (c=1, m=4)
+    X = mlp_ref.features(Q, [(4, 4)])[0]
+    s_E = abs(1.0 + 4 * b) + 2.0 + 1e-12
+    s_mx = 0.0 + 2.0 + 1e-12
+    expect = np.zeros(192)
+    expect[1 * 36 + 3 * 9 + 0] = np.float32(b / s_E)
+    expect[2 * 36 + 3 * 9 + 0] = np.float32(-b / s_E)
+    expect[0 * 36 + 1 * 9 + 4] = np.float32(0.5 / s_mx)
+    np.testing.assert_array_equal(X, expect)
