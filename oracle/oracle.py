@@ -17,6 +17,15 @@ LIB = HERE / "liboracle.so"
 _lib = None
 
 
+def use_library(path) -> None:
+    """Load the oracle from another build of zk_oracle.c (for instance one
+    compiled with -march=native for CPU timing); before the first call only."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    LIB = Path(path)
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:

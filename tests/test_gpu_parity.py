@@ -743,3 +743,38 @@ def test_ensemble_graph_replays_match_eager():
             Ue, te = ens[1].get(k)
             assert tg == te
             np.testing.assert_array_equal(Ug, Ue)
+
+
+@pytest.mark.parametrize("P,n,steps,seed", [(3, 128, 250, None), (2, 96, 200, 8), (3, 64, 120, 11)])
+def test_dg_tvb_lane_kernel_matches_scan_and_oracle(P, n, steps, seed, monkeypatch):
+    """The one-lane-per-cell TVB pass (default) against the 16-cell scan
+    (ZK_TVB_LANE=0, read at handle creation): state and flags bit for bit over
+    a run that limits many cells and certifies most of them, and both within
+    the parity bar of the oracle."""
+    say(f"dg P={P} tvb lane kernel vs scan, {n}^2, {steps} steps")
+    m = meshio.riemann_2d(n) if seed is None else meshio.random_riemann(seed, n)
+    runs = []
+    for lane in ("1", "0"):
+        monkeypatch.setenv("ZK_TVB_LANE", lane)
+        s = zk.Solver(method=zk.METHOD_DG, P=P, limiter=zk.LIMITER_TVB, tvb_M=50.0)
+        monkeypatch.delenv("ZK_TVB_LANE")
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+        runs.append(s)
+    dt = 0.9 * oracle.dg_cfl_limit(P) * m.da / runs[0].max_speed()
+    o = oracle.DG.from_prims(P, m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype, limiter=zk.LIMITER_TVB, tvb_M=50.0)
+    limited = 0
+    for _ in range(steps // 10):
+        for s in runs:
+            s.step(dt, 10)
+        o.step(dt, 10)
+        fa, fb = runs[0].get_flags(), runs[1].get_flags()
+        np.testing.assert_array_equal(fa, fb)
+        np.testing.assert_array_equal(runs[0].get_state(), runs[1].get_state())
+        limited += int(fa.sum())
+    Q = runs[0].get_state()
+    for c in range(4):
+        assert rel_l1(Q[c], o.Q[c]) <= TOL, c
+    np.testing.assert_array_equal(runs[0].get_flags(), o.flags)
+    assert limited > 0
+    for s in runs:
+        s.close()

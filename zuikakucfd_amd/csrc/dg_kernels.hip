@@ -1898,6 +1898,80 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
     }
 }
 
+// TVB limiter after the fused stage, one lane per cell.  A wave takes 64
+// consecutive cells of the update range and reads their marks.  A certified
+// cell (bit 0: its positivity bounds clear the floors) needs no positivity
+// scaling even when the TVB test limits it, because limiting keeps the means
+// and never raises the magnitude of a mode (zeroed modes; a TVB-modified
+// minmod of the linear modes), so the certificate's bounds only tighten.  Such
+// a cell that is also an updated candidate (bit 1) gets the TVB test in its own
+// lane, from the modes the test reads (coalesced across the wave), and a
+// limited variable gets its linear modes replaced and its other modes zeroed;
+// every other certified cell gets flag 0.  The few uncertified cells are then
+// taken 16 at a time by the matrix-core TVB + positivity routine, as the scan
+// does.  Results equal the scan's, and the oracle's TVB-then-positivity pass.
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+                                                          const int32_t *__restrict__ src,
+                                                          const int32_t *__restrict__ bct,
+                                                          const uint8_t *__restrict__ upd,
+                                                          uint8_t *__restrict__ flags,
+                                                          const uint8_t *__restrict__ zsok, int nx, int nxu,
+                                                          size_t ncell, size_t np, double Mh2, double gamma,
+                                                          const StepScalars *__restrict__ S, RingArgs R,
+                                                          unsigned nscan)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    if (S->done) return;  // uniform over the grid
+    if (blockIdx.x >= nscan) {
+        ring_item(R, (size_t)(blockIdx.x - nscan) * blockDim.x + threadIdx.x);
+        return;
+    }
+    const int lane = threadIdx.x & 63, col = lane & 15;
+    const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    if (t0 >= ncell) return;  // whole wave
+    const size_t t = t0 + lane;
+    bool defer = false;
+    size_t k = 0;
+    if (t < ncell) {
+        k = interior_cell(t, nx, nxu);
+        const unsigned z = zsok[k];
+        defer = !(z & 1);
+        if (!defer) {
+            bool lim = false;
+            if ((z & 2) && upd[k]) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    // the modes the test reads: the mean and those of degree 0 on one axis
+                    double Cc[NM];
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) {
+                        const int px = m / NQ, py = m % NQ;
+                        Cc[m] = (px == 0 || py == 0) && (P >= 3 || px + py <= 1)
+                                    ? Q[(size_t)(c * NM + m) * np + k]
+                                    : 0.0;
+                    }
+                    double axl, ayl;
+                    if (tvb_var<P>(Cc, Mn, src, np, k, nx, c, Mh2, axl, ayl)) {
+                        lim = true;
+#pragma unroll
+                        for (int m = 1; m < NM; ++m)
+                            Q[(size_t)(c * NM + m) * np + k] = m == NQ ? axl : (m == 1 ? ayl : 0.0);
+                    }
+                }
+            }
+            flags[k] = (uint8_t)lim;
+        }
+    }
+    const unsigned long long mask = __ballot(defer);
+    const int n = __popcll(mask);
+    for (int g0 = 0; g0 < n; g0 += 16) {
+        const int want = g0 + col < n ? g0 + col : n - 1;
+        const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Learned troubled-cell indicator (P = 2)
 // ---------------------------------------------------------------------------
@@ -2283,7 +2357,12 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // after the fused stage (zsok written): a per-thread screen lists the cells that
         // are limited or not certified, and the matrix-core kernel treats only those;
         // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
-        if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
+        if (A.zsok && !H.no_mfma && H.tvb_lane) {  // one lane per cell; matrix-core routine for the uncertified
+            const unsigned nscan = (unsigned)((ncell + 255) / 256);
+            hipLaunchKernelGGL((dg_tvb_lane_kernel<P>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256), 0,
+                               H.stream, Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell,
+                               H.g.np, Mh2, H.g.gamma, H.scal, RA, nscan);
+        } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper
             constexpr int CW = P == 3 ? 32 : 64;

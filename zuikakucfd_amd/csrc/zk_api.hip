@@ -156,6 +156,10 @@ void drop_graph(Handle *H)
     if (H->step_graph) (void)hipGraphDestroy(H->step_graph);
     H->step_exec = nullptr;
     H->step_graph = nullptr;
+    if (H->cfl_exec) (void)hipGraphExecDestroy(H->cfl_exec);
+    if (H->cfl_graph) (void)hipGraphDestroy(H->cfl_graph);
+    H->cfl_exec = nullptr;
+    H->cfl_graph = nullptr;
 }
 
 void free_all(Handle *H)
@@ -296,6 +300,20 @@ zk_status enqueue_step(Handle *H)
 
 // Enqueues nsteps steps with the dt in device memory: one captured step
 // replayed, or the launches themselves.
+// One step of CFL stepping: the wave-speed maximum, dt = cfl * da / lambda
+// (clipped to t_end; done once there), the three stages, and the time
+// advance, all from device memory.
+zk_status enqueue_cfl_step(Handle *H)
+{
+    if (H->cfg.method == ZK_METHOD_WENO) zk::fv_max_speed(*H, H->q);
+    else zk::dg_max_speed(*H, H->q);
+    hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, H->lam_bits, H->g.da);
+    const zk_status st = enqueue_step(H);
+    if (st) return st;
+    hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, H->stream, H->scal);
+    return ZK_OK;
+}
+
 zk_status enqueue_steps(Handle *H, int nsteps)
 {
     zk_status st = ZK_OK;
@@ -383,10 +401,11 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
     // measurement overrides: ZK_NO_MFMA=1 (VALU cell kernel), ZK_DMA=0..3 (memory pipeline of
     // the fused DG kernel's workgroup form), ZK_MF4=0 (its 16x16x4 products instead of the
-    // 4x4x4 blocks), ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is
-    // ignored)
+    // 4x4x4 blocks), ZK_TVB_LANE=0 (the 16-cell TVB scan instead of one lane per cell),
+    // ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is ignored)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
     if (const char *v = std::getenv("ZK_MF4")) H->mf4 = v[0] != '0';
+    if (const char *v = std::getenv("ZK_TVB_LANE")) H->tvb_lane = v[0] != '0';
     if (const char *v = std::getenv("ZK_DMA"))
         if (v[0] >= '0' && v[0] <= '3' && v[1] == 0) H->dma = v[0] - '0';
     if (const char *v = std::getenv("ZK_TUNE")) {
@@ -689,21 +708,36 @@ zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps, double
     const int steps0 = s.steps;
     ZK_HIP(hipMemcpyAsync(H->scal, &s, sizeof(s), hipMemcpyHostToDevice, H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
-    for (int k = 0; k < nsteps; ++k) {
-        if (H->cfg.method == ZK_METHOD_WENO) zk::fv_max_speed(*H, H->q);
-        else zk::dg_max_speed(*H, H->q);
-        hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(64), 0, H->stream, H->scal, H->lam_bits, H->g.da);
-        // stage kernels read dt from device memory; a step after t_end has
-        // dt = 0 and done = 1, which the host detects below and stops.
-        if ((k & 15) == 15 || k == nsteps - 1) {
-            ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
-            ZK_HIP(hipStreamSynchronize(H->stream));
-            if (s.done) break;
+    // One CFL step (wave speed, dt, the three stages, time advance) reads and
+    // writes only device memory; it is captured once and replayed like
+    // zk_step's, in chunks of 16 between host reads of the step scalars.  A
+    // step after t_end (or after a wave speed that is not finite) has done = 1
+    // and dt = 0: its stage kernels return at once, and the host stops.
+    const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
+    const bool graph = !salted && H->use_graph && !H->prof;
+    if (graph && !H->cfl_exec) {
+        ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
+        st = enqueue_cfl_step(H);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(H->stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
         }
-        st = enqueue_step(H);
-        if (st) return st;
-        hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, H->stream, H->scal);
-        ++H->steps_enqueued;
+        ZK_HIP(e);
+        H->cfl_graph = g;
+        ZK_HIP(hipGraphInstantiate(&H->cfl_exec, g, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < nsteps; k += 16) {
+        const int chunk = nsteps - k < 16 ? nsteps - k : 16;
+        for (int c = 0; c < chunk; ++c) {
+            if (graph) ZK_HIP(hipGraphLaunch(H->cfl_exec, H->stream));
+            else if ((st = enqueue_cfl_step(H)) != ZK_OK) return st;
+        }
+        H->steps_enqueued += chunk;
+        ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
+        ZK_HIP(hipStreamSynchronize(H->stream));
+        if (s.done) break;
     }
     ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));

@@ -56,10 +56,13 @@ struct Handle {
                               // 2 tables in LDS and rows by global-to-LDS loads, 3 the same with two
                               // row buffers (ZK_DMA overrides)
     bool mf4 = true;          // fused DG kernel (two row buffers) on 4x4x4 matrix-core blocks (ZK_MF4=0: 16x16x4)
+    bool tvb_lane = true;     // TVB after the fused kernel: one lane per cell (ZK_TVB_LANE=0: the 16-cell scan)
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
     bool use_graph = true;    // WENO zk_step: replay one captured step as a HIP graph
     hipGraph_t step_graph = nullptr;
     hipGraphExec_t step_exec = nullptr;
+    hipGraph_t cfl_graph = nullptr;      // one CFL step: wave speed, dt, three stages, time advance
+    hipGraphExec_t cfl_exec = nullptr;
     int64_t n_updated = 0;
 
     // device buffers
