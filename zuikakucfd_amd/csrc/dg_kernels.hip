@@ -610,6 +610,28 @@ __device__ __forceinline__ void face_devs(const double q[P], double &fp, double 
 // changes one of the face-average deviations xp, xm, yp, ym.  Leaves the
 // limited linear modes in axl/ayl and returns whether the variable is
 // troubled.  Neighbour means are read only where a value is above M h^2.
+// tvb_axes with the four neighbour means given (east j+1, west j-1, south
+// i+1, north i-1): the same arithmetic, for callers that load the means first.
+__device__ __forceinline__ bool tvb_axes_nb(double ub, double ax, double ay, bool faces, double xp, double xm,
+                                            double yp, double ym, double mE, double mW, double mS, double mN,
+                                            double Mh2, double &axl, double &ayl)
+{
+    bool trouble = false;
+    axl = ax;
+    ayl = ay;
+    if (!(fabs(ax) <= Mh2) || (faces && (!(fabs(xp) <= Mh2) || !(fabs(xm) <= Mh2)))) {
+        const double dp = mE - ub, dm = ub - mW;
+        axl = tvb_mod(ax, dp, dm, Mh2);
+        if (faces) trouble = tvb_mod(xp, dp, dm, Mh2) != xp || tvb_mod(xm, dp, dm, Mh2) != xm;
+    }
+    if (!(fabs(ay) <= Mh2) || (faces && (!(fabs(yp) <= Mh2) || !(fabs(ym) <= Mh2)))) {
+        const double dp = mS - ub, dm = ub - mN;
+        ayl = tvb_mod(ay, dp, dm, Mh2);
+        if (faces) trouble = trouble || tvb_mod(yp, dp, dm, Mh2) != yp || tvb_mod(ym, dp, dm, Mh2) != ym;
+    }
+    return trouble || axl != ax || ayl != ay;
+}
+
 __device__ __forceinline__ bool tvb_axes(double ub, double ax, double ay, bool faces, double xp, double xm,
                                          double yp, double ym, const double *__restrict__ M,
                                          const int32_t *__restrict__ src, size_t np, size_t k, int nx, int c,
@@ -1907,19 +1929,22 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
 // a cell that is also an updated candidate (bit 1) gets the TVB test in its own
 // lane, from the modes the test reads (coalesced across the wave), and a
 // limited variable gets its linear modes replaced and its other modes zeroed;
-// every other certified cell gets flag 0.  The few uncertified cells are then
-// taken 16 at a time by the matrix-core TVB + positivity routine, as the scan
-// does.  Results equal the scan's, and the oracle's TVB-then-positivity pass.
+// every other certified cell gets flag 0.  The few uncertified cells are
+// listed (dcount[stage], dcells; one atomic per wave) for dg_tvb_defer_kernel,
+// which gives them the matrix-core TVB + positivity routine; keeping that
+// routine out of this kernel keeps its registers low, so more waves hide the
+// latency of the scattered neighbour-mean loads.  Results equal the 16-cell
+// scan's, and the oracle's TVB-then-positivity pass.
 template <int P>
 __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
-                                                          const int32_t *__restrict__ bct,
                                                           const uint8_t *__restrict__ upd,
                                                           uint8_t *__restrict__ flags,
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
-                                                          size_t ncell, size_t np, double Mh2, double gamma,
+                                                          size_t ncell, size_t np, double Mh2,
                                                           const StepScalars *__restrict__ S, RingArgs R,
-                                                          unsigned nscan)
+                                                          unsigned nscan, unsigned *__restrict__ dcount,
+                                                          int32_t *__restrict__ dcells)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     if (S->done) return;  // uniform over the grid
@@ -1927,7 +1952,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
         ring_item(R, (size_t)(blockIdx.x - nscan) * blockDim.x + threadIdx.x);
         return;
     }
-    const int lane = threadIdx.x & 63, col = lane & 15;
+    const int lane = threadIdx.x & 63;
     const size_t t0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
     if (t0 >= ncell) return;  // whole wave
     const size_t t = t0 + lane;
@@ -1940,35 +1965,84 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
         if (!defer) {
             bool lim = false;
             if ((z & 2) && upd[k]) {
+                // every load first (the stores below would otherwise order them):
+                // the modes the test reads (the mean and the modes of degree 0 on
+                // one axis, x = (q,0), y = (0,q)) and the four neighbour means
+                constexpr int NT = P >= 3 ? P : 1;
+                double ub[4], X[4][NT], Y[4][NT], mE[4], mW[4], mS[4], mN[4];
+                const int32_t eE = src[k + 1], eW = src[k - 1], eS = src[k + nx], eN = src[k - nx];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    // the modes the test reads: the mean and those of degree 0 on one axis
-                    double Cc[NM];
+                    const double *q = Q + (size_t)(c * NM) * np + k;
+                    ub[c] = q[0];
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) {
-                        const int px = m / NQ, py = m % NQ;
-                        Cc[m] = (px == 0 || py == 0) && (P >= 3 || px + py <= 1)
-                                    ? Q[(size_t)(c * NM + m) * np + k]
-                                    : 0.0;
+                    for (int d = 1; d <= NT; ++d) {
+                        X[c][d - 1] = q[(size_t)(d * NQ) * np];
+                        Y[c][d - 1] = q[(size_t)d * np];
                     }
-                    double axl, ayl;
-                    if (tvb_var<P>(Cc, Mn, src, np, k, nx, c, Mh2, axl, ayl)) {
-                        lim = true;
+                    auto nbm = [&](int32_t e) {
+                        double v = Mn[(size_t)c * np + (size_t)(e >> kSrcShift)];
+                        if ((c == 1 && (e & 1)) || (c == 2 && (e & 2))) v = -v;
+                        return v;
+                    };
+                    mE[c] = nbm(eE);
+                    mW[c] = nbm(eW);
+                    mS[c] = nbm(eS);
+                    mN[c] = nbm(eN);
+                }
+                bool chg[4];
+                double axl[4], ayl[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    double xp = X[c][0], xm = X[c][0], yp = Y[c][0], ym = Y[c][0];
+                    if constexpr (P >= 3) {
+                        face_devs<P>(X[c], xp, xm);
+                        face_devs<P>(Y[c], yp, ym);
+                    }
+                    chg[c] = tvb_axes_nb(ub[c], X[c][0], Y[c][0], P >= 3, xp, xm, yp, ym, mE[c], mW[c], mS[c], mN[c],
+                                         Mh2, axl[c], ayl[c]);
+                    lim = lim || chg[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (chg[c])
 #pragma unroll
                         for (int m = 1; m < NM; ++m)
-                            Q[(size_t)(c * NM + m) * np + k] = m == NQ ? axl : (m == 1 ? ayl : 0.0);
-                    }
-                }
+                            Q[(size_t)(c * NM + m) * np + k] = m == NQ ? axl[c] : (m == 1 ? ayl[c] : 0.0);
             }
             flags[k] = (uint8_t)lim;
         }
     }
     const unsigned long long mask = __ballot(defer);
-    const int n = __popcll(mask);
-    for (int g0 = 0; g0 < n; g0 += 16) {
-        const int want = g0 + col < n ? g0 + col : n - 1;
-        const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n);
+    if (!mask) return;
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(dcount, (unsigned)__popcll(mask));
+    base = __shfl(base, leader);
+    if (defer) dcells[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
+}
+
+// The uncertified cells the lane kernel listed (dcount[slot] of them): the
+// matrix-core TVB + positivity routine, 16 cells per wave per pass, over a
+// grid-strided range.  Block 0 clears the next stage's count (the counts
+// rotate over the three stages, so a captured step needs no memset).
+template <int P>
+__global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
+                                                           const int32_t *__restrict__ src,
+                                                           const int32_t *__restrict__ bct,
+                                                           uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
+                                                           double gamma, const StepScalars *__restrict__ S,
+                                                           unsigned *__restrict__ dcount, int slot,
+                                                           const int32_t *__restrict__ dcells)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) dcount[(slot + 1) % 3] = 0u;
+    if (S->done) return;  // uniform over the grid
+    const unsigned n = dcount[slot];
+    const int col = threadIdx.x & 15;
+    const unsigned waves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u; g0 < n; g0 += waves * 16u) {
+        const unsigned i = g0 + col < n ? g0 + col : n - 1;  // padding lanes repeat the last cell, store nothing
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, (size_t)dcells[i], g0 + col < n);
     }
 }
 
@@ -2357,11 +2431,19 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // after the fused stage (zsok written): a per-thread screen lists the cells that
         // are limited or not certified, and the matrix-core kernel treats only those;
         // otherwise the matrix-core kernel at P = 3 and the per-thread kernel at P <= 2
-        if (A.zsok && !H.no_mfma && H.tvb_lane) {  // one lane per cell; matrix-core routine for the uncertified
+        if (A.zsok && !H.no_mfma && H.tvb_lane && H.dcount) {
+            // one lane per cell, then the matrix-core routine for the uncertified cells
             const unsigned nscan = (unsigned)((ncell + 255) / 256);
-            hipLaunchKernelGGL((dg_tvb_lane_kernel<P>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256), 0,
-                               H.stream, Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell,
-                               H.g.np, Mh2, H.g.gamma, H.scal, RA, nscan);
+            {
+                ProfScope pl(H, "dg_tvb_lane");
+                hipLaunchKernelGGL((dg_tvb_lane_kernel<P>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256), 0,
+                                   H.stream, Qout, H.means, H.src, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell,
+                                   H.g.np, Mh2, H.scal, RA, nscan, H.dcount + stage, H.dcells);
+            }
+            const unsigned nblk = (unsigned)(H.ncu > 0 ? H.ncu : 256);
+            ProfScope pd(H, "dg_tvb_defer");
+            hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.means, H.src,
+                               H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.dcount, stage, H.dcells);
         } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper

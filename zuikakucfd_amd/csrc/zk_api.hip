@@ -167,7 +167,7 @@ void free_all(Handle *H)
     drop_graph(H);
     void *ptrs[] = {H->q, H->qa, H->qb, H->src, H->srcy, H->upd, H->ring, H->gx, H->gy, H->means, H->flags,
                     H->scal, H->lam_bits, H->health, H->bct, H->stats, H->score, H->cand, H->bad, H->any, H->blist,
-                    H->zsok, H->tlist};
+                    H->zsok, H->tlist, H->dcount, H->dcells};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     H->q = H->qa = H->qb = nullptr;
@@ -175,6 +175,7 @@ void free_all(Handle *H)
     H->gx = H->gy = H->means = nullptr; H->flags = nullptr;
     H->scal = nullptr; H->lam_bits = nullptr; H->health = nullptr; H->bct = nullptr;
     H->stats = nullptr; H->score = nullptr; H->cand = nullptr; H->bad = nullptr; H->any = nullptr; H->blist = nullptr; H->zsok = nullptr; H->tlist = nullptr;
+    H->dcount = nullptr; H->dcells = nullptr;
     H->loaded = false;
 }
 
@@ -540,6 +541,11 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
         ZK_HIP(hipMalloc(&H->means, 4 * np * sizeof(double)));
         ZK_HIP(hipMalloc(&H->zsok, np));
         ZK_HIP(hipMemset(H->zsok, 0, np));
+        if (H->cfg.limiter == ZK_LIMITER_TVB) {
+            ZK_HIP(hipMalloc(&H->dcount, 3 * sizeof(unsigned)));
+            ZK_HIP(hipMemset(H->dcount, 0, 3 * sizeof(unsigned)));
+            ZK_HIP(hipMalloc(&H->dcells, (nxu * nyu + 1) * sizeof(int32_t)));
+        }
         if (H->cfg.limiter == ZK_LIMITER_LEARNED) {
             ZK_HIP(hipMalloc(&H->tlist, (1 + nxu * nyu) * sizeof(int32_t)));
             ZK_HIP(hipMemset(H->tlist, 0, sizeof(int32_t)));

@@ -104,6 +104,8 @@ struct Handle {
     int2 *blist = nullptr;        // the bad cells of the current stage
     uint8_t *zsok = nullptr;      // DG TVB: cells whose positivity scaling is certified a no-op
     int32_t *tlist = nullptr;     // DG learned: [0] = count, then the cells the limiter pass treats
+    unsigned *dcount = nullptr;   // DG TVB: uncertified cells the lane pass deferred, one count per stage
+    int32_t *dcells = nullptr;    // DG TVB: those cells
     int ncu = 0;              // compute units of the device
 
     // kernel timing: when prof is set every launcher brackets its kernel with
