@@ -35,6 +35,7 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <initializer_list>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -159,44 +160,62 @@ struct Opts {
 
 bool parse_opts(int argc, char **argv, Opts &o)
 {
-    for (int a = 4; a < argc; ++a) {
-        std::string k = argv[a];
-        auto val = [&](std::string &v) {
-            if (a + 1 >= argc) return false;
-            v = argv[++a];
-            return true;
-        };
-        std::string v;
-        if (k == "--flux" && val(v)) o.flux = v == "hll" ? ZK_FLUX_HLL : ZK_FLUX_HLLC;
-        else if (k == "--limiter" && val(v))
-            o.limiter = v == "tvb" ? ZK_LIMITER_TVB : v == "none" ? ZK_LIMITER_NONE
-                        : v == "learned" ? ZK_LIMITER_LEARNED : ZK_LIMITER_PERSSON_ZS;
-        else if (k == "--M" && val(v)) o.M = std::stod(v);
-        else if (k == "--precision" && val(v)) o.precision = v == "fp32" ? ZK_PREC_FP32_BULK : ZK_PREC_FP64;
-        else if (k == "--cfl" && val(v)) o.cfl = std::stod(v);
-        else if (k == "--weights" && val(v)) o.weights = v;
-        else if (k == "--band" && a + 2 < argc) {
-            o.band_lo = std::stof(argv[++a]);
-            o.band_hi = std::stof(argv[++a]);
-        } else if (k == "--audit" && val(v)) o.audit = std::stof(v);
-        else if (k == "--ranks" && val(v)) o.ranks = std::max(1, std::stoi(v));
-        else if (k == "--dump") o.dump = true;
-        else if (k == "--quiet") o.quiet = true;
-        else if (k == "--positivity") o.positivity = true;
-        else if (k == "--slip") o.slip = true;
-        else if (k == "--recon" && val(v)) {
-            if (v == "weno5") o.recon = ZK_RECON_WENO5;
-            else if (v == "muscl-minmod") o.recon = ZK_RECON_MUSCL_MINMOD;
-            else if (v == "muscl-vanleer") o.recon = ZK_RECON_MUSCL_VANLEER;
-            else if (v == "muscl-superbee") o.recon = ZK_RECON_MUSCL_SUPERBEE;
-            else {
-                std::cerr << "unknown reconstruction " << v << "\n";
+    // an option value that is not one of the listed names is an error (exit 1)
+    auto pick = [](const std::string &what, const std::string &v,
+                   std::initializer_list<std::pair<const char *, int>> names, int &out) {
+        for (const auto &n : names)
+            if (v == n.first) {
+                out = n.second;
+                return true;
+            }
+        std::cerr << "unknown " << what << " " << v << "\n";
+        return false;
+    };
+    try {
+        for (int a = 4; a < argc; ++a) {
+            std::string k = argv[a];
+            auto val = [&](std::string &v) {
+                if (a + 1 >= argc) return false;
+                v = argv[++a];
+                return true;
+            };
+            std::string v;
+            if (k == "--flux" && val(v)) {
+                if (!pick("flux", v, {{"hllc", ZK_FLUX_HLLC}, {"hll", ZK_FLUX_HLL}}, o.flux)) return false;
+            } else if (k == "--limiter" && val(v)) {
+                if (!pick("limiter", v,
+                          {{"persson", ZK_LIMITER_PERSSON_ZS}, {"tvb", ZK_LIMITER_TVB}, {"none", ZK_LIMITER_NONE},
+                           {"learned", ZK_LIMITER_LEARNED}},
+                          o.limiter))
+                    return false;
+            } else if (k == "--M" && val(v)) o.M = std::stod(v);
+            else if (k == "--precision" && val(v)) {
+                if (!pick("precision", v, {{"fp64", ZK_PREC_FP64}, {"fp32", ZK_PREC_FP32_BULK}}, o.precision)) return false;
+            } else if (k == "--cfl" && val(v)) o.cfl = std::stod(v);
+            else if (k == "--weights" && val(v)) o.weights = v;
+            else if (k == "--band" && a + 2 < argc) {
+                o.band_lo = std::stof(argv[++a]);
+                o.band_hi = std::stof(argv[++a]);
+            } else if (k == "--audit" && val(v)) o.audit = std::stof(v);
+            else if (k == "--ranks" && val(v)) o.ranks = std::max(1, std::stoi(v));
+            else if (k == "--dump") o.dump = true;
+            else if (k == "--quiet") o.quiet = true;
+            else if (k == "--positivity") o.positivity = true;
+            else if (k == "--slip") o.slip = true;
+            else if (k == "--recon" && val(v)) {
+                if (!pick("reconstruction", v,
+                          {{"weno5", ZK_RECON_WENO5}, {"muscl-minmod", ZK_RECON_MUSCL_MINMOD},
+                           {"muscl-vanleer", ZK_RECON_MUSCL_VANLEER}, {"muscl-superbee", ZK_RECON_MUSCL_SUPERBEE}},
+                          o.recon))
+                    return false;
+            } else {
+                std::cerr << "unknown option " << k << "\n";
                 return false;
             }
-        } else {
-            std::cerr << "unknown option " << k << "\n";
-            return false;
         }
+    } catch (const std::exception &) {
+        std::cerr << "bad numeric option value\n";
+        return false;
     }
     return true;
 }
