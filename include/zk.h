@@ -63,6 +63,9 @@ typedef int zk_status;
 #define ZK_LIMITER_TVB 2        /* Cockburn-Shu TVB (M; at P >= 3 also the face-average
                                    test) + Zhang-Shu positivity */
 #define ZK_LIMITER_LEARNED 3    /* MLP troubled-cell indicator gating the TVB limiter */
+#define ZK_LIMITER_TVB_LINEAR 4 /* ZK_LIMITER_TVB with the linear-mode test alone at every P (the rule
+                                   README_old.md:137-208 describes; at P >= 3 it lets the four-quadrant
+                                   problem blow up, which the face-average test of ZK_LIMITER_TVB prevents) */
 
 typedef struct zk_config {
     int method;      /* ZK_METHOD_* */

@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
         L.zko_dg_rhs.argtypes = [i, i, i, d, d, ip, dp, i, dp]
         L.zko_dg_limit_persson_zs.argtypes = [i, i, i, d, ip, dp]
         L.zko_dg_limit_tvb.argtypes = [i, i, i, d, d, ip, dp, up]
+        L.zko_dg_limit_tvb_linear.argtypes = [i, i, i, d, d, ip, dp, up]
         L.zko_dg_limit_zs.argtypes = [i, i, i, d, dp]
         L.zko_dg_step.argtypes = [i, i, i, d, d, ip, dp, d, i, i, d, up]
         L.zko_dg_max_speed.argtypes = [i, i, i, d, ip, dp]
@@ -177,9 +178,10 @@ class DG:
     def fill(self) -> None:
         lib().zko_dg_fill_ghosts(self.ny, self.nx, self.P, self.bct, self.Q)
 
-    def limit_tvb(self) -> np.ndarray:
+    def limit_tvb(self, linear_only: bool = False) -> np.ndarray:
         f = np.zeros((self.ny, self.nx), dtype=np.uint8)
-        lib().zko_dg_limit_tvb(self.ny, self.nx, self.P, self.da, self.M, self.bct, self.Q, f)
+        fn = lib().zko_dg_limit_tvb_linear if linear_only else lib().zko_dg_limit_tvb
+        fn(self.ny, self.nx, self.P, self.da, self.M, self.bct, self.Q, f)
         return f
 
     def limit_persson_zs(self) -> None:

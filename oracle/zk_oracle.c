@@ -945,9 +945,10 @@ static inline double tvb_minmod(double a, double b, double c, double Mh2)
  * differences as the linear mode of its axis.  The linear-mode test alone
  * leaves a cell with small linear but large quadratic and cubic modes
  * unlimited; at P = 3 that lets the four-quadrant problem blow up near
- * t = 0.0054. */
-void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t *bct,
-                      double *Q, uint8_t *flags)
+ * t = 0.0054.  faces = 0 gives the linear-mode test alone at every P (the rule
+ * README_old.md:137-208 describes; zko_dg_limit_tvb_linear). */
+static void limit_tvb(int ny, int nx, int P, double da, double M, const int32_t *bct,
+                      double *Q, uint8_t *flags, int faces)
 {
     const int nm = (P + 1) * (P + 1);
     const size_t NP = (size_t)ny * nx;
@@ -974,7 +975,7 @@ void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t 
                 double axl = tvb_minmod(ax, dxp, dxm, Mh2);
                 double ayl = tvb_minmod(ay, dyp, dym, Mh2);
                 int trouble = axl != ax || ayl != ay;
-                if (P >= 3) {
+                if (P >= 3 && faces) {
                     double e = 0.0, w = 0.0, n = 0.0, so = 0.0;
                     for (int q = 1; q <= P; ++q) {
                         const double qx = Q[(size_t)(c * nm + q * (P + 1)) * NP + k];
@@ -999,6 +1000,18 @@ void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t 
     free(A);
 }
 
+void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t *bct,
+                      double *Q, uint8_t *flags)
+{
+    limit_tvb(ny, nx, P, da, M, bct, Q, flags, 1);
+}
+
+void zko_dg_limit_tvb_linear(int ny, int nx, int P, double da, double M, const int32_t *bct,
+                             double *Q, uint8_t *flags)
+{
+    limit_tvb(ny, nx, P, da, M, bct, Q, flags, 0);
+}
+
 /* SSP-RK3 on all modes with the limiter after every stage.
  * src_DG/fluid.cpp:1005-1054. */
 void zko_dg_step(int ny, int nx, int P, double da, double gamma, const int32_t *bct,
@@ -1018,8 +1031,8 @@ void zko_dg_step(int ny, int nx, int P, double da, double gamma, const int32_t *
         else
             for (size_t k = 0; k < N; ++k) Q[k] = (1.0 / 3.0) * Qn[k] + (2.0 / 3.0) * (Q[k] + dt * dQ[k]);
         if (limiter == 1) zko_dg_limit_persson_zs(ny, nx, P, gamma, bct, Q);
-        else if (limiter == 2) {
-            zko_dg_limit_tvb(ny, nx, P, da, tvb_M, bct, Q, flags);
+        else if (limiter == 2 || limiter == 4) {
+            limit_tvb(ny, nx, P, da, tvb_M, bct, Q, flags, limiter == 2);
             zko_dg_limit_zs(ny, nx, P, gamma, Q);
         }
     }

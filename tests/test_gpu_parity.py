@@ -778,3 +778,29 @@ def test_dg_tvb_lane_kernel_matches_scan_and_oracle(P, n, steps, seed, monkeypat
     assert limited > 0
     for s in runs:
         s.close()
+
+
+def test_dg3_tvb_linear_only_parity():
+    """ZK_LIMITER_TVB_LINEAR (the linear-mode test alone) at P = 3 against the
+    oracle's limiter 4 over 100 steps of the four-quadrant problem at 64^2,
+    state and flags; it limits fewer cells than the default rule."""
+    say("dg P=3 tvb linear-only 64^2 x100")
+    m = meshio.riemann_2d(64)
+    s, o = dg_pair(m, 3, zk.LIMITER_TVB_LINEAR, M=50.0)
+    d = zk.Solver(method=zk.METHOD_DG, P=3, limiter=zk.LIMITER_TVB, tvb_M=50.0)
+    d.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+    dt = oracle.dg_cfl_limit(3) * m.da / o.max_speed()
+    n_lin = n_face = 0
+    for _ in range(10):
+        s.step(dt, 10)
+        d.step(dt, 10)
+        o.step(dt, 10)
+        np.testing.assert_array_equal(s.get_flags(), o.flags)
+        n_lin += int(o.flags.sum())
+        n_face += int(d.get_flags().sum())
+    Q = s.get_state()
+    for c in range(4):
+        assert rel_l1(Q[c], o.Q[c]) <= TOL, c
+    assert n_lin < n_face
+    s.close()
+    d.close()

@@ -519,6 +519,45 @@ def test_dg3_tvb_four_quadrant_stays_finite():
     assert o.Q[0, 0][upd].min() > 0 and o.Q[3, 0][upd].min() > 0
 
 
+def test_tvb_linear_only_rule():
+    """limiter 4 (linear modes alone): the first cell of the face-average test
+    above (W deviation out of bounds, linear modes within) is left alone at
+    P = 3, and at P = 1, 2 the two rules agree exactly."""
+    n, h = 14, 0.1
+    bct = meshio.ring_bctype(n, n)
+    j = np.arange(n)
+    Q = np.zeros((4, 16, n, n))
+    Q[:, 0] = 1.0 + 0.01 * j[None, None, :]
+    Q[:, 4] = 0.005
+    Q[:, 4, 6, 6], Q[:, 8, 6, 6] = 0.008, -0.004
+    o = oracle.DG(3, h, G, Q, bct, tvb_M=0.0)
+    assert o.limit_tvb()[6, 6] == 1
+    o = oracle.DG(3, h, G, Q, bct, tvb_M=0.0)
+    assert o.limit_tvb(linear_only=True)[6, 6] == 0
+    np.testing.assert_array_equal(o.Q[:, :, 6, 6], Q[:, :, 6, 6])
+    m = meshio.riemann_2d(24)
+    for P in (1, 2):
+        a = oracle.DG.from_prims(P, m.da, G, m.rho, m.u, m.v, m.p, m.bctype, limiter=2, tvb_M=50.0)
+        b = oracle.DG.from_prims(P, m.da, G, m.rho, m.u, m.v, m.p, m.bctype, limiter=4, tvb_M=50.0)
+        dt = oracle.dg_cfl_limit(P) * m.da / a.max_speed()
+        a.step(dt, 20)
+        b.step(dt, 20)
+        np.testing.assert_array_equal(a.Q, b.Q)
+        np.testing.assert_array_equal(a.flags, b.flags)
+
+
+def test_dg3_tvb_linear_only_four_quadrant_blows_up():
+    """The same 48^2 run with the linear-mode rule alone goes non-finite
+    between steps 240 and 250: the reason the face-average test exists."""
+    m = meshio.riemann_2d(48)
+    o = oracle.DG.from_prims(3, m.da, G, m.rho, m.u, m.v, m.p, m.bctype, limiter=4, tvb_M=50.0)
+    dt = oracle.dg_cfl_limit(3) * m.da / o.max_speed()
+    o.step(dt, 230)
+    assert np.isfinite(o.Q).all()
+    o.step(dt, 30)
+    assert not np.isfinite(o.Q).all()
+
+
 def test_persson_closed_forms():
     P, n = 2, 8
     nm = 9

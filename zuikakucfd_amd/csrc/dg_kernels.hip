@@ -657,7 +657,7 @@ __device__ __forceinline__ bool tvb_axes(double ub, double ax, double ay, bool f
 template <int P>
 __device__ __forceinline__ bool tvb_var(const double Cc[(P + 1) * (P + 1)], const double *__restrict__ M,
                                         const int32_t *__restrict__ src, size_t np, size_t k, int nx, int c,
-                                        double Mh2, double &axl, double &ayl)
+                                        double Mh2, double &axl, double &ayl, bool lin = false)
 {
     constexpr int NQ = P + 1;
     const double ax = Cc[NQ], ay = Cc[1];
@@ -672,7 +672,7 @@ __device__ __forceinline__ bool tvb_var(const double Cc[(P + 1) * (P + 1)], cons
         face_devs<P>(qx, xp, xm);
         face_devs<P>(qy, yp, ym);
     }
-    return tvb_axes(Cc[0], ax, ay, P >= 3, xp, xm, yp, ym, M, src, np, k, nx, c, Mh2, axl, ayl);
+    return tvb_axes(Cc[0], ax, ay, P >= 3 && !lin, xp, xm, yp, ym, M, src, np, k, nx, c, Mh2, axl, ayl);
 }
 
 // Component-wise TVB limiter (Cockburn & Shu 2001, modified minmod with
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
                                                      const int32_t *__restrict__ src,
                                                      const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                      int ny, int nx, size_t np, double Mh2, double gamma,
-                                                     const StepScalars *__restrict__ S)
+                                                     const StepScalars *__restrict__ S, bool lin)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const int nxu = nx - 6, nyu = ny - 6;
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             double axl, ayl;
-            if (tvb_var<P>(C[c], M, src, np, k, nx, c, Mh2, axl, ayl)) {
+            if (tvb_var<P>(C[c], M, src, np, k, nx, c, Mh2, axl, ayl, lin)) {
 #pragma unroll
                 for (int m = 1; m < NM; ++m) C[c][m] = 0.0;
                 C[c][mxm] = axl;
@@ -1647,7 +1647,8 @@ template <int P>
 __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const double *__restrict__ Mn,
                                                const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
                                                uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
-                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live)
+                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live,
+                                               bool lin)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const MfmaTabsF &MT = c_mff[P - 1];
@@ -1685,7 +1686,8 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
         }
         double a_x = ax, a_y = ay_in;
         if (g == 0 && tvb_cell) {
-            if (tvb_axes(ub, ax, ay_in, P >= 3, xp, xm, yp, ym, Mn, src, np, k, nx, c, Mh2, a_x, a_y)) chg |= 1 << c;
+            if (tvb_axes(ub, ax, ay_in, P >= 3 && !lin, xp, xm, yp, ym, Mn, src, np, k, nx, c, Mh2, a_x, a_y))
+                chg |= 1 << c;
         }
         axl[c] = __shfl(a_x, col);
         ayl[c] = __shfl(a_y, col);
@@ -1803,7 +1805,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
                                                           const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                           int ny, int nx, size_t np, double Mh2, double gamma,
                                                           const uint8_t *__restrict__ zsok,
-                                                          const StepScalars *__restrict__ S)
+                                                          const StepScalars *__restrict__ S, bool lin)
 {
     if (S->done) return;
     const int nxu = nx - 6, nyu = ny - 6;
@@ -1814,7 +1816,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     const bool live = t0 + col < ncell;
     const size_t t = live ? t0 + col : ncell - 1;
     const size_t k = interior_cell(t, nx, nxu);
-    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live);
+    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live, lin);
 }
 
 // Lists the cells the limiter pass must treat, from the byte the fused stage
@@ -1889,7 +1891,7 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
                                                           size_t ncell, size_t np, double Mh2, double gamma,
                                                           const StepScalars *__restrict__ S, RingArgs R,
-                                                          unsigned nscan)
+                                                          unsigned nscan, bool lin)
 {
     if (S->done) return;  // uniform over the grid
     if (blockIdx.x >= nscan) {
@@ -1916,7 +1918,7 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
         // column col of this group takes listed cell g0 + col (padding repeats the last one)
         const int want = g0 + col < n ? g0 + col : n - 1;
         const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n);
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n, lin);
     }
 }
 
@@ -1944,7 +1946,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
                                                           size_t ncell, size_t np, double Mh2,
                                                           const StepScalars *__restrict__ S, RingArgs R,
                                                           unsigned nscan, unsigned *__restrict__ dcount,
-                                                          int32_t *__restrict__ dcells)
+                                                          int32_t *__restrict__ dcells, bool lin)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     if (S->done) return;  // uniform over the grid
@@ -1999,8 +2001,8 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
                         face_devs<P>(X[c], xp, xm);
                         face_devs<P>(Y[c], yp, ym);
                     }
-                    chg[c] = tvb_axes_nb(ub[c], X[c][0], Y[c][0], P >= 3, xp, xm, yp, ym, mE[c], mW[c], mS[c], mN[c],
-                                         Mh2, axl[c], ayl[c]);
+                    chg[c] = tvb_axes_nb(ub[c], X[c][0], Y[c][0], P >= 3 && !lin, xp, xm, yp, ym, mE[c], mW[c],
+                                         mS[c], mN[c], Mh2, axl[c], ayl[c]);
                     lim = lim || chg[c];
                 }
 #pragma unroll
@@ -2033,7 +2035,7 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
                                                            uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
                                                            double gamma, const StepScalars *__restrict__ S,
                                                            unsigned *__restrict__ dcount, int slot,
-                                                           const int32_t *__restrict__ dcells)
+                                                           const int32_t *__restrict__ dcells, bool lin)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) dcount[(slot + 1) % 3] = 0u;
     if (S->done) return;  // uniform over the grid
@@ -2042,7 +2044,8 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
     const unsigned waves = gridDim.x * (blockDim.x >> 6);
     for (unsigned g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u; g0 < n; g0 += waves * 16u) {
         const unsigned i = g0 + col < n ? g0 + col : n - 1;  // padding lanes repeat the last cell, store nothing
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, (size_t)dcells[i], g0 + col < n);
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, (size_t)dcells[i], g0 + col < n,
+                          lin);
     }
 }
 
@@ -2438,12 +2441,13 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
                 ProfScope pl(H, "dg_tvb_lane");
                 hipLaunchKernelGGL((dg_tvb_lane_kernel<P>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256), 0,
                                    H.stream, Qout, H.means, H.src, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell,
-                                   H.g.np, Mh2, H.scal, RA, nscan, H.dcount + stage, H.dcells);
+                                   H.g.np, Mh2, H.scal, RA, nscan, H.dcount + stage, H.dcells, H.tvb_linear);
             }
             const unsigned nblk = (unsigned)(H.ncu > 0 ? H.ncu : 256);
             ProfScope pd(H, "dg_tvb_defer");
             hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.means, H.src,
-                               H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.dcount, stage, H.dcells);
+                               H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.dcount, stage, H.dcells,
+                               H.tvb_linear);
         } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper
@@ -2452,14 +2456,14 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256),
                                0, H.stream,
                                Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
-                               H.g.gamma, H.scal, RA, nscan);
+                               H.g.gamma, H.scal, RA, nscan, H.tvb_linear);
         } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal);
+                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.tvb_linear);
         else
             hipLaunchKernelGGL((dg_tvb_mfma_kernel<P>), dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, H.stream,
                                Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, A.zsok,
-                               H.scal);
+                               H.scal, H.tvb_linear);
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {

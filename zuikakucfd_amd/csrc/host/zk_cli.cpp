@@ -185,7 +185,7 @@ bool parse_opts(int argc, char **argv, Opts &o)
             } else if (k == "--limiter" && val(v)) {
                 if (!pick("limiter", v,
                           {{"persson", ZK_LIMITER_PERSSON_ZS}, {"tvb", ZK_LIMITER_TVB}, {"none", ZK_LIMITER_NONE},
-                           {"learned", ZK_LIMITER_LEARNED}},
+                           {"learned", ZK_LIMITER_LEARNED}, {"tvb-linear", ZK_LIMITER_TVB_LINEAR}},
                           o.limiter))
                     return false;
             } else if (k == "--M" && val(v)) o.M = std::stod(v);

@@ -375,7 +375,7 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
         g_create_err = "unknown flux";
         return ZK_ERR_ARG;
     }
-    if (cfg->method == ZK_METHOD_DG && (cfg->limiter < 0 || cfg->limiter > ZK_LIMITER_LEARNED)) {
+    if (cfg->method == ZK_METHOD_DG && (cfg->limiter < 0 || cfg->limiter > ZK_LIMITER_TVB_LINEAR)) {
         g_create_err = "unknown limiter";
         return ZK_ERR_ARG;
     }
@@ -399,6 +399,11 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     }
     Handle *H = new Handle();
     H->cfg = *cfg;
+    if (cfg->method == ZK_METHOD_DG && cfg->limiter == ZK_LIMITER_TVB_LINEAR) {
+        // the TVB path with the linear-mode test alone (the face-average test off)
+        H->cfg.limiter = ZK_LIMITER_TVB;
+        H->tvb_linear = true;
+    }
     H->nm = cfg->method == ZK_METHOD_DG ? (cfg->P + 1) * (cfg->P + 1) : 1;
     // measurement overrides: ZK_NO_MFMA=1 (VALU cell kernel), ZK_DMA=0..3 (memory pipeline of
     // the fused DG kernel's workgroup form), ZK_MF4=0 (its 16x16x4 products instead of the

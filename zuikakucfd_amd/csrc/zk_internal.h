@@ -57,6 +57,7 @@ struct Handle {
                               // row buffers (ZK_DMA overrides)
     bool mf4 = true;          // fused DG kernel (two row buffers) on 4x4x4 matrix-core blocks (ZK_MF4=0: 16x16x4)
     bool tvb_lane = true;     // TVB after the fused kernel: one lane per cell (ZK_TVB_LANE=0: the 16-cell scan)
+    bool tvb_linear = false;  // ZK_LIMITER_TVB_LINEAR: the TVB test on the linear modes alone at every P
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
     bool use_graph = true;    // WENO zk_step: replay one captured step as a HIP graph
     hipGraph_t step_graph = nullptr;

@@ -20,7 +20,7 @@ LIB_PATH = _PKG / "libzk.so"
 METHOD_WENO, METHOD_DG = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
 FLUX_HLLC, FLUX_HLL = 0, 1
-LIMITER_NONE, LIMITER_PERSSON_ZS, LIMITER_TVB, LIMITER_LEARNED = 0, 1, 2, 3
+LIMITER_NONE, LIMITER_PERSSON_ZS, LIMITER_TVB, LIMITER_LEARNED, LIMITER_TVB_LINEAR = 0, 1, 2, 3, 4
 RECON_WENO5, RECON_MUSCL_MINMOD, RECON_MUSCL_VANLEER, RECON_MUSCL_SUPERBEE = 0, 1, 2, 3
 
 ZK_OK = 0
