@@ -75,13 +75,17 @@ typedef struct zk_config {
     int limiter;     /* ZK_LIMITER_*; DG only */
     double tvb_M;    /* TVB constant M (50 in the benchmark configurations) */
     int device;      /* HIP device ordinal */
-    int positivity;  /* WENO: 1 = positivity option (FP64), no reference counterpart;
-                        0 reproduces the reference scheme.  Face check: a face whose
-                        reconstructed states have rho or p below 1e-12 uses the
-                        first-order states.  Cell fallback: an updated cell whose new
-                        state has rho or p not above 1e-12 (or is not finite) is bad, and
-                        every face touching a bad cell takes the first-order flux for the
-                        stage (once; conservative, faces shared). */
+    int positivity;  /* 1 = positivity option (FP64), no reference counterpart; 0
+                        reproduces the reference scheme.
+                        WENO: face check: a face whose reconstructed states have rho or p
+                        below 1e-12 uses the first-order states.  Cell fallback: an updated
+                        cell whose new state has rho or p not above 1e-12 (or is not finite)
+                        is bad, and every face touching a bad cell takes the first-order
+                        flux for the stage (once; conservative, faces shared).
+                        DG (Zhang-Shu scaling of every limiter): the minima of rho and p
+                        are also taken at the face quadrature points the fluxes read and
+                        at the cell corners the CFL rule reads, with the full basis,
+                        after the reference's 4x4 points (src_DG/fluid.cpp:892-991). */
 } zk_config;
 
 /* Create a handle.  Replaces the `Mesh` object's lifetime

@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         L.zko_dg_limit_tvb.argtypes = [i, i, i, d, d, ip, dp, up]
         L.zko_dg_limit_tvb_linear.argtypes = [i, i, i, d, d, ip, dp, up]
         L.zko_dg_limit_zs.argtypes = [i, i, i, d, dp]
+        L.zko_dg_limit_zs_ext.argtypes = [i, i, i, d, dp]
         L.zko_dg_step.argtypes = [i, i, i, d, d, ip, dp, d, i, i, d, up]
         L.zko_dg_max_speed.argtypes = [i, i, i, d, ip, dp]
         L.zko_dg_max_speed.restype = d
@@ -99,6 +100,8 @@ def prim_to_cons(gamma, rho, u, v, p) -> np.ndarray:
 
 
 POSITIVITY = 16
+# DG limiter bit: the positivity scaling also at the face points and corners (zk_oracle.h)
+DG_POSITIVITY = 8
 # flux bits selecting MUSCL instead of WENO5 (zk_oracle.h ZKO_RECON_SHIFT)
 MUSCL_MINMOD, MUSCL_VANLEER, MUSCL_SUPERBEE = 1 << 5, 2 << 5, 3 << 5
 # flux bit making bctype-1 cells mirror slip walls (zk_oracle.h ZKO_FV_SLIP)
@@ -187,8 +190,8 @@ class DG:
     def limit_persson_zs(self) -> None:
         lib().zko_dg_limit_persson_zs(self.ny, self.nx, self.P, self.gamma, self.bct, self.Q)
 
-    def limit_zs(self) -> None:
-        lib().zko_dg_limit_zs(self.ny, self.nx, self.P, self.gamma, self.Q)
+    def limit_zs(self, ext: bool = False) -> None:
+        (lib().zko_dg_limit_zs_ext if ext else lib().zko_dg_limit_zs)(self.ny, self.nx, self.P, self.gamma, self.Q)
 
     def max_speed(self) -> float:
         return lib().zko_dg_max_speed(self.ny, self.nx, self.P, self.gamma, self.bct, self.Q)

@@ -814,10 +814,31 @@ static inline double zs_basis(int P, int m, double xi, double eta)
 static const double ZS_PTS[4] = {-0.8611363115940526, -0.3399810435848563,
                                  0.3399810435848563, 0.8611363115940526};
 
-/* Zhang-Shu style positivity scaling on every cell of [3,ny-3)x[3,nx-3)
- * (no bctype test).  src_DG/fluid.cpp:892-991. */
-void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q)
+/* Points of the positivity option (no reference counterpart): the face
+ * quadrature points the fluxes read, (-1, eta_q), (+1, eta_q), (xi_q, -1),
+ * (xi_q, +1), and the four corners (+-1, +-1) the CFL rule reads, with the
+ * full Legendre basis.  Returns their count, 4 (P+1) + 4. */
+static int ext_points(int P, double xs[], double ys[])
 {
+    const int NQ = P + 1;
+    const double *gp = GLP[NQ];
+    int n = 0;
+    for (int q = 0; q < NQ; ++q) { xs[n] = -1.0; ys[n++] = gp[q]; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = +1.0; ys[n++] = gp[q]; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = gp[q]; ys[n++] = -1.0; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = gp[q]; ys[n++] = +1.0; }
+    for (int c = 0; c < 4; ++c) { xs[n] = (c & 1) ? 1.0 : -1.0; ys[n++] = (c & 2) ? 1.0 : -1.0; }
+    return n;
+}
+
+/* Zhang-Shu style positivity scaling on every cell of [3,ny-3)x[3,nx-3)
+ * (no bctype test).  src_DG/fluid.cpp:892-991.  ext = 1 (the positivity
+ * option) also takes both minima over the points of ext_points, with the
+ * full basis, after the reference's 16 points. */
+static void limit_zs(int ny, int nx, int P, double gamma, double *Q, int ext)
+{
+    double xs[20], ys[20];
+    const int nx_ext = ext ? ext_points(P, xs, ys) : 0;
     const int nm = (P + 1) * (P + 1);
     const size_t NP = (size_t)ny * nx;
     const double RHO_MIN = 1e-5, P_MIN = 1e-5, EPS = 1e-5;
@@ -843,6 +864,11 @@ void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q)
                         theta_rho = stdmin(theta_rho, t);
                     }
                 }
+            for (int e = 0; e < nx_ext; e++) {
+                double rho = 0.0;
+                for (int m = 0; m < nm; m++) rho += D(0, m) * phi2(P, m, xs[e], ys[e]);
+                if (rho < RHO_MIN) theta_rho = stdmin(theta_rho, (rho0 - RHO_MIN) / (rho0 - rho + EPS));
+            }
             if (theta_rho < 1.0)
                 for (int m = 1; m < nm; m++)
                     for (int c = 0; c < 4; ++c) D(c, m) *= theta_rho;
@@ -867,6 +893,23 @@ void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q)
                         theta_p = stdmin(theta_p, t);
                     }
                 }
+            for (int e = 0; e < nx_ext; e++) {
+                double rho = 0.0, mx = 0.0, my = 0.0, E = 0.0;
+                for (int m = 0; m < nm; m++) {
+                    double phi = phi2(P, m, xs[e], ys[e]);
+                    rho += D(0, m) * phi;
+                    mx += D(1, m) * phi;
+                    my += D(2, m) * phi;
+                    E += D(3, m) * phi;
+                }
+                if (rho <= RHO_MIN) continue;
+                double u = mx / rho, v = my / rho;
+                double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+                if (p < P_MIN) {
+                    double p_avg = (gamma - 1.0) * (D(3, 0) - 0.5 * (D(1, 0) * D(1, 0) + D(2, 0) * D(2, 0)) / D(0, 0));
+                    theta_p = stdmin(theta_p, (p_avg - P_MIN) / (p_avg - p + EPS));
+                }
+            }
             if (theta_p < 1.0)
                 for (int m = 1; m < nm; m++)
                     for (int c = 0; c < 4; ++c) D(c, m) *= theta_p;
@@ -874,10 +917,19 @@ void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q)
 #undef D
 }
 
+void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q)
+{
+    limit_zs(ny, nx, P, gamma, Q, 0);
+}
+
+void zko_dg_limit_zs_ext(int ny, int nx, int P, double gamma, double *Q)
+{
+    limit_zs(ny, nx, P, gamma, Q, 1);
+}
+
 /* Persson modal-decay filter on bctype-0 cells of [3,ny-3)x[3,nx-3), then the
  * positivity scaling.  src_DG/fluid.cpp:824-887 and 892-991. */
-void zko_dg_limit_persson_zs(int ny, int nx, int P, double gamma, const int32_t *bct,
-                             double *Q)
+static void limit_persson_zs(int ny, int nx, int P, double gamma, const int32_t *bct, double *Q, int ext)
 {
     const int nm = (P + 1) * (P + 1);
     const size_t NP = (size_t)ny * nx;
@@ -912,7 +964,13 @@ void zko_dg_limit_persson_zs(int ny, int nx, int P, double gamma, const int32_t 
                 }
             }
         }
-    zko_dg_limit_zs(ny, nx, P, gamma, Q);
+    limit_zs(ny, nx, P, gamma, Q, ext);
+}
+
+void zko_dg_limit_persson_zs(int ny, int nx, int P, double gamma, const int32_t *bct,
+                             double *Q)
+{
+    limit_persson_zs(ny, nx, P, gamma, bct, Q, 0);
 }
 
 static inline double minmod3(double a, double b, double c)
@@ -1030,10 +1088,11 @@ void zko_dg_step(int ny, int nx, int P, double da, double gamma, const int32_t *
             for (size_t k = 0; k < N; ++k) Q[k] = 0.75 * Qn[k] + 0.25 * (Q[k] + dt * dQ[k]);
         else
             for (size_t k = 0; k < N; ++k) Q[k] = (1.0 / 3.0) * Qn[k] + (2.0 / 3.0) * (Q[k] + dt * dQ[k]);
-        if (limiter == 1) zko_dg_limit_persson_zs(ny, nx, P, gamma, bct, Q);
-        else if (limiter == 2 || limiter == 4) {
-            limit_tvb(ny, nx, P, da, tvb_M, bct, Q, flags, limiter == 2);
-            zko_dg_limit_zs(ny, nx, P, gamma, Q);
+        const int lim = limiter & 7, ext = (limiter & 8) != 0;
+        if (lim == 1) limit_persson_zs(ny, nx, P, gamma, bct, Q, ext);
+        else if (lim == 2 || lim == 4) {
+            limit_tvb(ny, nx, P, da, tvb_M, bct, Q, flags, lim == 2);
+            limit_zs(ny, nx, P, gamma, Q, ext);
         }
     }
     free(Qn);

@@ -72,7 +72,9 @@ void zko_dg_rhs(int ny, int nx, int P, double da, double gamma, const int32_t *b
 /* limiter: 0 none, 1 Persson + Zhang-Shu (the reference's applyLimiter),
  * 2 TVB (Cockburn-Shu, modified minmod with constant M; at P >= 3 with the
  * face-average test) + Zhang-Shu positivity, 4 the same with the linear-mode
- * test alone at every P.
+ * test alone at every P; or-ed with 8 (the positivity option, no reference
+ * counterpart) the positivity scaling also covers the face quadrature points
+ * and the cell corners, with the full basis (zko_dg_limit_zs_ext).
  * flags (optional, ny*nx bytes) receive 1 where the TVB limiter changed a cell. */
 void zko_dg_limit_persson_zs(int ny, int nx, int P, double gamma, const int32_t *bct,
                              double *Q);
@@ -81,6 +83,7 @@ void zko_dg_limit_tvb(int ny, int nx, int P, double da, double M, const int32_t 
 void zko_dg_limit_tvb_linear(int ny, int nx, int P, double da, double M, const int32_t *bct,
                              double *Q, uint8_t *flags);
 void zko_dg_limit_zs(int ny, int nx, int P, double gamma, double *Q);
+void zko_dg_limit_zs_ext(int ny, int nx, int P, double gamma, double *Q);
 void zko_dg_step(int ny, int nx, int P, double da, double gamma, const int32_t *bct,
                  double *Q, double dt, int flux, int limiter, double tvb_M,
                  uint8_t *flags);

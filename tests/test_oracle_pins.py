@@ -270,3 +270,78 @@ def test_features_slot_order():
     expect[2 * 36 + 3 * 9 + 0] = np.float32(-b / s_E)
     expect[0 * 36 + 1 * 9 + 4] = np.float32(0.5 / s_mx)
     np.testing.assert_array_equal(X, expect)
+
+
+# --------------------------------------------------------------------------
+# Positivity option of the DG path (no reference counterpart): the scaling's
+# minima also over the face quadrature points and the corners, full basis
+# --------------------------------------------------------------------------
+def test_zs_ext_face_point_closed_form():
+    """P = 1, mean density 1, x-linear density mode 1.05: the reference's
+    4x4 points see rho >= 1 - 1.05 * 0.8611 > 0 and leave the cell alone; the
+    option sees rho = -0.05 at xi = -1 (face points and corners) and scales the
+    modes by (1 - 1e-5) / (1 + 0.05 + 1e-5)."""
+    Q, bct = cell_state(1)
+    Q[0, 2, 4, 4] = 1.05
+    o = oracle.DG(1, 0.1, G, Q, bct)
+    o.limit_zs()
+    np.testing.assert_array_equal(o.Q, Q)
+    o = oracle.DG(1, 0.1, G, Q, bct)
+    o.limit_zs(ext=True)
+    theta = (1.0 - 1e-5) / (1.0 + 0.05 + 1e-5)
+    assert o.Q[0, 2, 4, 4] == pytest.approx(1.05 * theta, rel=1e-14)
+
+
+def test_zs_ext_uses_the_full_basis_at_p3():
+    """P = 3, a (3,0) density mode of 1.5: the reference's truncated basis
+    ignores it; at the face points xi = -1 and the corners the full basis gives
+    rho = 1 - 1.5 = -0.5, the minimum, so theta = (1 - 1e-5) / (1.5 + 1e-5)."""
+    Q, bct = cell_state(3)
+    Q[0, 12, 4, 4] = 1.5
+    o = oracle.DG(3, 0.1, G, Q, bct)
+    o.limit_zs()
+    np.testing.assert_array_equal(o.Q, Q)
+    o.limit_zs(ext=True)
+    others = min(1.0 + 1.5 * leg(3, x) for x in (1.0,) + GP4)   # E face and the N/S face points
+    assert others > -0.5
+    theta = (1.0 - 1e-5) / (1.0 + 0.5 + 1e-5)
+    assert o.Q[0, 12, 4, 4] == pytest.approx(1.5 * theta, rel=1e-14)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_zs_ext_makes_every_point_positive(P):
+    """Random cells with positive means and large higher modes: after the
+    scaling with the option, density and pressure exceed 1e-5 (to rounding) at
+    the 4x4 points (truncated basis) and at every face point and corner (full
+    basis).  Pressure is concave in the conserved variables, so the linear
+    theta is enough."""
+    rng = np.random.default_rng(P)
+    n = 12
+    nm = (P + 1) ** 2
+    Q = np.zeros((4, nm, n, n))
+    rho = rng.uniform(0.2, 2.0, (n, n)); u = rng.uniform(-1, 1, (n, n)); v = rng.uniform(-1, 1, (n, n))
+    p = rng.uniform(0.05, 2.0, (n, n))
+    Q[:, 0] = oracle.prim_to_cons(G, rho, u, v, p)
+    Q[:, 1:] = rng.normal(size=(4, nm - 1, n, n)) * 0.6 * np.abs(Q[:, :1])
+    o = oracle.DG(P, 0.1, G, Q, meshio.ring_bctype(n, n))
+    o.limit_zs(ext=True)
+    gp = np.polynomial.legendre.leggauss(P + 1)[0]
+    pts = [(-1.0, y) for y in gp] + [(1.0, y) for y in gp] + [(x, -1.0) for x in gp] + [(x, 1.0) for x in gp]
+    pts += [(sx, sy) for sx in (-1.0, 1.0) for sy in (-1.0, 1.0)]
+
+    def value(i, j, x, y, full):
+        out = np.zeros(4)
+        for m in range(nm):
+            px, py = divmod(m, P + 1)
+            bx = leg(px, x) if (full or px < 3) else 0.0
+            by = leg(py, y) if (full or py < 3) else 0.0
+            out += o.Q[:, m, i, j] * bx * by
+        return out
+
+    for i in range(3, n - 3):
+        for j in range(3, n - 3):
+            for (x, y, full) in [(a, b, False) for a in GP4 for b in GP4] + [(a, b, True) for a, b in pts]:
+                U = value(i, j, x, y, full)
+                pr = (G - 1) * (U[3] - 0.5 * (U[1] ** 2 + U[2] ** 2) / U[0])
+                assert U[0] > 1e-5 * (1 - 1e-9), (i, j, x, y)
+                assert pr > 1e-5 * (1 - 1e-6), (i, j, x, y, pr)

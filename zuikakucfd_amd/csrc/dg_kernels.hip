@@ -73,8 +73,26 @@ struct DgTables {
     double ww[DG_MAXNQ][DG_MAXNQ];                   // w_a * w_b
     double corner[4][DG_MAXNM];                      // phi_m at (+-1, +-1) for the CFL speed
     double zs[4][4][DG_MAXNM];                       // truncated basis() at the 4x4 ZS grid
+    double zx[20][DG_MAXNM];                         // positivity option: full basis at the face points and
+                                                     // corners (4 NQ + 4 of them; the rest repeat the first)
     double invm[DG_MAXNM];                           // (2px+1)(2py+1)
 };
+
+// The extra points of the positivity option (no reference counterpart): the
+// face quadrature points (-1, eta_q), (+1, eta_q), (xi_q, -1), (xi_q, +1) that
+// the fluxes read and the corners (+-1, +-1) that the CFL rule reads.
+inline int zs_ext_points(int P, const double *gp, double xs[20], double ys[20])
+{
+    const int NQ = P + 1;
+    int n = 0;
+    for (int q = 0; q < NQ; ++q) { xs[n] = -1.0; ys[n++] = gp[q]; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = +1.0; ys[n++] = gp[q]; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = gp[q]; ys[n++] = -1.0; }
+    for (int q = 0; q < NQ; ++q) { xs[n] = gp[q]; ys[n++] = +1.0; }
+    for (int c = 0; c < 4; ++c) { xs[n] = (c & 1) ? 1.0 : -1.0; ys[n++] = (c & 2) ? 1.0 : -1.0; }
+    for (int e = n; e < 20; ++e) { xs[e] = xs[0]; ys[e] = ys[0]; }
+    return n;
+}
 
 __constant__ DgTables c_tab[3];  // index P-1
 
@@ -182,6 +200,9 @@ DgTables make_tables(int P)
         for (int k = 0; k < 4; ++k) T.corner[k][m] = legP_h(px, XI[k]) * legP_h(py, ETA[k]);
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b) T.zs[a][b][m] = zsP_h(px, zsp[a]) * zsP_h(py, zsp[b]);
+        double xs[20], ys[20];
+        zs_ext_points(P, gp, xs, ys);
+        for (int e = 0; e < 20; ++e) T.zx[e][m] = legP_h(px, xs[e]) * legP_h(py, ys[e]);
     }
     for (int a = 0; a < NQ; ++a)
         for (int b = 0; b < NQ; ++b) T.ww[a][b] = gw[a] * gw[b];
@@ -242,6 +263,7 @@ struct DgArgs {
     double da, gamma;
     double ca, cb;
     int stage, flux, limiter;
+    int zs_ext;       // positivity option: the scaling also at the face points and corners
 };
 
 template <int P, bool X>
@@ -343,8 +365,9 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
 
 // Positivity scaling of one cell (src_DG/fluid.cpp:897-991).
 template <int P>
-__device__ __forceinline__ bool zs_cell(double Q[4][(P + 1) * (P + 1)], double gamma)
+__device__ __forceinline__ bool zs_cell(double Q[4][(P + 1) * (P + 1)], double gamma, bool ext = false)
 {
+    constexpr int NX = 4 * (P + 1) + 4;  // extra points of the positivity option
     bool changed = false;
     constexpr int NM = (P + 1) * (P + 1);
     const DgTables &T = c_tab[P - 1];
@@ -369,6 +392,13 @@ __device__ __forceinline__ bool zs_cell(double Q[4][(P + 1) * (P + 1)], double g
                 const double t = (rho0 - RHO_MIN) / (rho0 - rho + EPS);
                 theta_rho = smin(theta_rho, t);
             }
+        }
+    if (ext)
+        for (int e = 0; e < NX; e++) {
+            double rho = 0.0;
+#pragma unroll
+            for (int m = 0; m < NM; m++) rho += Q[0][m] * T.zx[e][m];
+            if (rho < RHO_MIN) theta_rho = smin(theta_rho, (rho0 - RHO_MIN) / (rho0 - rho + EPS));
         }
     if (theta_rho < 1.0) {
         changed = true;
@@ -397,6 +427,25 @@ __device__ __forceinline__ bool zs_cell(double Q[4][(P + 1) * (P + 1)], double g
                 const double p_avg = (gamma - 1.0) * (Q[3][0] - 0.5 * (Q[1][0] * Q[1][0] + Q[2][0] * Q[2][0]) / Q[0][0]);
                 const double t = (p_avg - P_MIN) / (p_avg - p + EPS);
                 theta_p = smin(theta_p, t);
+            }
+        }
+    if (ext)
+        for (int e = 0; e < NX; e++) {
+            double rho = 0.0, mx = 0.0, my = 0.0, E = 0.0;
+#pragma unroll
+            for (int m = 0; m < NM; m++) {
+                const double phi = T.zx[e][m];
+                rho += Q[0][m] * phi;
+                mx += Q[1][m] * phi;
+                my += Q[2][m] * phi;
+                E += Q[3][m] * phi;
+            }
+            if (rho <= RHO_MIN) continue;
+            const double u = mx / rho, v = my / rho;
+            const double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+            if (p < P_MIN) {
+                const double p_avg = (gamma - 1.0) * (Q[3][0] - 0.5 * (Q[1][0] * Q[1][0] + Q[2][0] * Q[2][0]) / Q[0][0]);
+                theta_p = smin(theta_p, (p_avg - P_MIN) / (p_avg - p + EPS));
             }
         }
     if (theta_p < 1.0) {
@@ -548,7 +597,7 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
         }
     if (A.limiter == ZK_LIMITER_PERSSON_ZS) {
         if (A.bct[k] == 0) persson_cell<P>(Q);
-        zs_cell<P>(Q, A.gamma);
+        zs_cell<P>(Q, A.gamma, A.zs_ext != 0);
     } else if (A.means) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) A.means[(size_t)c * np + k] = Q[c][0];
@@ -685,7 +734,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
                                                      const int32_t *__restrict__ src,
                                                      const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                      int ny, int nx, size_t np, double Mh2, double gamma,
-                                                     const StepScalars *__restrict__ S, bool lin)
+                                                     const StepScalars *__restrict__ S, bool lin, bool ext)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const int nxu = nx - 6, nyu = ny - 6;
@@ -712,7 +761,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
             }
         }
     }
-    const bool changed = zs_cell<P>(C, gamma);
+    const bool changed = zs_cell<P>(C, gamma, ext);
     flags[k] = (uint8_t)flag;
     if (flag || changed) {
 #pragma unroll
@@ -963,8 +1012,17 @@ struct MfmaTabsF {
     double ayA[64];      // y-face projection: row i -> w_q L_{i/4}(s_q)
     double zsA[4][64];   // positivity points: truncated basis() at (a, b), point l%16 = 4a + b
     double zc[16];       // slot 4px+py: max over the 16 positivity points of |truncated basis()|
+    double zcx[16];      // the same over those and the extra points of the positivity option (full basis there)
 };
 __constant__ MfmaTabsF c_mff[3];
+
+// The positivity option's extra points (zs_ext_points) for the matrix-core
+// positivity routine: chain h evaluates points 16 h + l % 16 (padded with the
+// first point) of mode (px = s, py = l / 16) with the full basis.
+struct MfmaTabsX {
+    double zxA[2][4][64];
+};
+__constant__ MfmaTabsX c_mfx[3];
 
 MfmaTabsF make_mfma_tables_fused(int P);
 
@@ -985,6 +1043,7 @@ struct MfmaTabs4 {
     double aWL[64];     // A[py][q] = w_q L_py(eta_q): x-face and volume F projection onto py
     double aWD[64];     // A[py][q] = w_q L'_py(eta_q): volume G projection onto py
     double zc[16];      // as MfmaTabsF::zc
+    double zcx[16];     // as MfmaTabsF::zcx
 };
 __constant__ MfmaTabs4 c_mf4[3];
 
@@ -1009,7 +1068,10 @@ MfmaTabs4 make_mfma_tables4(int P)
         M.aWD[l] = wt(r) * D(c, loc(r));
     }
     const MfmaTabsF F = make_mfma_tables_fused(P);
-    for (int i = 0; i < 16; ++i) M.zc[i] = F.zc[i];
+    for (int i = 0; i < 16; ++i) {
+        M.zc[i] = F.zc[i];
+        M.zcx[i] = F.zcx[i];
+    }
     return M;
 }
 
@@ -1068,6 +1130,11 @@ MfmaTabsF make_mfma_tables_fused(int P)
                 for (int a = 0; a < 4; ++a)
                     for (int b = 0; b < 4; ++b) mx = std::max(mx, std::fabs(zsP_h(px, zsp[a]) * zsP_h(py, zsp[b])));
             M.zc[4 * px + py] = mx;
+            double xs[20], ys[20];
+            const int ne = zs_ext_points(P, gp, xs, ys);
+            for (int e = 0; e < ne && px <= P && py <= P; ++e)
+                mx = std::max(mx, std::fabs(legP_h(px, xs[e]) * legP_h(py, ys[e])));
+            M.zcx[4 * px + py] = mx;
         }
     return M;
 }
@@ -1393,7 +1460,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                         mean0[c] = out;
                         if (A.means) A.means[(size_t)c * np + kp] = out;
                     } else {
-                        sab[c] += fabs(out) * (F4 ? M4.zc[4 * v + g] : M.zc[4 * v + g]);
+                        const double *zc = A.zs_ext ? (F4 ? M4.zcx : M.zcx) : (F4 ? M4.zc : M.zc);
+                        sab[c] += fabs(out) * zc[4 * v + g];
                     }
                     if constexpr (P >= 3) {
                         if (v == 0)
@@ -1648,10 +1716,12 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
                                                const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
                                                uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
                                                double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live,
-                                               bool lin)
+                                               bool lin, bool ext = false)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const MfmaTabsF &MT = c_mff[P - 1];
+    const MfmaTabsX &MX = c_mfx[P - 1];
+    constexpr int NXCH = (4 * NQ + 4 + 15) / 16;  // chains of 16 extra points (positivity option)
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const bool slot = g <= P;
     const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
@@ -1748,6 +1818,16 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
 #pragma unroll
         for (int v = 0; v < 4; ++v)
             if (z[v] < RHO_MIN) th = smin(th, (m0[0] - RHO_MIN) / (m0[0] - z[v] + EPS));
+        if (ext)
+#pragma unroll
+            for (int h = 0; h < NXCH; ++h) {
+                d4 zx = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s <= P; ++s) zx = __builtin_amdgcn_mfma_f64_16x16x4f64(MX.zxA[h][s][lane], C[0][s], zx, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (zx[v] < RHO_MIN) th = smin(th, (m0[0] - RHO_MIN) / (m0[0] - zx[v] + EPS));
+            }
         th = smin(th, __shfl_xor(th, 16));
         th = smin(th, __shfl_xor(th, 32));
         if (th < 1.0) {
@@ -1771,14 +1851,31 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
         }
         const double p_avg = (gamma - 1.0) * (m0[3] - 0.5 * (m0[1] * m0[1] + m0[2] * m0[2]) / m0[0]);
         double th = 1.0;
+        auto pmin = [&](const d4 zz[4]) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double rho = z[0][v];
-            if (rho <= RHO_MIN) continue;
-            const double u = z[1][v] / rho, w = z[2][v] / rho;
-            const double p = (gamma - 1.0) * (z[3][v] - 0.5 * rho * (u * u + w * w));
-            if (p < P_MIN) th = smin(th, (p_avg - P_MIN) / (p_avg - p + EPS));
-        }
+            for (int v = 0; v < 4; ++v) {
+                const double rho = zz[0][v];
+                if (rho <= RHO_MIN) continue;
+                const double u = zz[1][v] / rho, w = zz[2][v] / rho;
+                const double p = (gamma - 1.0) * (zz[3][v] - 0.5 * rho * (u * u + w * w));
+                if (p < P_MIN) th = smin(th, (p_avg - P_MIN) / (p_avg - p + EPS));
+            }
+        };
+        pmin(z);
+        if (ext)
+#pragma unroll
+            for (int h = 0; h < NXCH; ++h) {
+                d4 zx[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    d4 a = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s <= P; ++s)
+                        a = __builtin_amdgcn_mfma_f64_16x16x4f64(MX.zxA[h][s][lane], C[c][s], a, 0, 0, 0);
+                    zx[c] = a;
+                }
+                pmin(zx);
+            }
         th = smin(th, __shfl_xor(th, 16));
         th = smin(th, __shfl_xor(th, 32));
         if (th < 1.0) {
@@ -1805,7 +1902,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
                                                           const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
                                                           int ny, int nx, size_t np, double Mh2, double gamma,
                                                           const uint8_t *__restrict__ zsok,
-                                                          const StepScalars *__restrict__ S, bool lin)
+                                                          const StepScalars *__restrict__ S, bool lin, bool ext)
 {
     if (S->done) return;
     const int nxu = nx - 6, nyu = ny - 6;
@@ -1816,7 +1913,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     const bool live = t0 + col < ncell;
     const size_t t = live ? t0 + col : ncell - 1;
     const size_t k = interior_cell(t, nx, nxu);
-    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live, lin);
+    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live, lin, ext);
 }
 
 // Lists the cells the limiter pass must treat, from the byte the fused stage
@@ -1891,7 +1988,7 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
                                                           size_t ncell, size_t np, double Mh2, double gamma,
                                                           const StepScalars *__restrict__ S, RingArgs R,
-                                                          unsigned nscan, bool lin)
+                                                          unsigned nscan, bool lin, bool ext)
 {
     if (S->done) return;  // uniform over the grid
     if (blockIdx.x >= nscan) {
@@ -1918,7 +2015,7 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
         // column col of this group takes listed cell g0 + col (padding repeats the last one)
         const int want = g0 + col < n ? g0 + col : n - 1;
         const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n, lin);
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n, lin, ext);
     }
 }
 
@@ -2035,7 +2132,7 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
                                                            uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
                                                            double gamma, const StepScalars *__restrict__ S,
                                                            unsigned *__restrict__ dcount, int slot,
-                                                           const int32_t *__restrict__ dcells, bool lin)
+                                                           const int32_t *__restrict__ dcells, bool lin, bool ext)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) dcount[(slot + 1) % 3] = 0u;
     if (S->done) return;  // uniform over the grid
@@ -2045,7 +2142,7 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
     for (unsigned g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u; g0 < n; g0 += waves * 16u) {
         const unsigned i = g0 + col < n ? g0 + col : n - 1;  // padding lanes repeat the last cell, store nothing
         tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, (size_t)dcells[i], g0 + col < n,
-                          lin);
+                          lin, ext);
     }
 }
 
@@ -2164,7 +2261,7 @@ __device__ __forceinline__ void learned_cell(double *__restrict__ Q, const doubl
                                              size_t np, double Mh2, double gamma, float lo, float hi,
                                              uint32_t audit_thresh, uint32_t salt,
                                              unsigned long long *__restrict__ stats,
-                                             const uint8_t *__restrict__ zsok, size_t k)
+                                             const uint8_t *__restrict__ zsok, size_t k, bool ext)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     double C[4][NM];
@@ -2226,7 +2323,7 @@ __device__ __forceinline__ void learned_cell(double *__restrict__ Q, const doubl
             flag = 1;
         }
     }
-    const bool changed = zs_cell<P>(C, gamma);
+    const bool changed = zs_cell<P>(C, gamma, ext);
     flags[k] = (uint8_t)flag;
     if (flag || changed) {
 #pragma unroll
@@ -2253,7 +2350,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
                                                          unsigned long long *__restrict__ stats,
                                                          const uint8_t *__restrict__ zsok,
                                                          const int32_t *__restrict__ list,
-                                                         const StepScalars *__restrict__ S)
+                                                         const StepScalars *__restrict__ S, bool ext)
 {
     if (S->done) return;
     const int nxu = nx - 6, nyu = ny - 6;
@@ -2261,7 +2358,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
         const size_t k = LIST ? (size_t)list[1 + t] : interior_cell(t, nx, nxu);
         learned_cell<P>(Q, M, src, bct, flags, score, nx, np, Mh2, gamma, lo, hi, audit_thresh, salt, stats,
-                        LIST ? nullptr : zsok, k);
+                        LIST ? nullptr : zsok, k, ext);
     }
 }
 
@@ -2327,6 +2424,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.cb = stage == 1 ? 0.25 : (2.0 / 3.0);
     A.flux = H.cfg.flux;
     A.limiter = H.cfg.limiter;
+    A.zs_ext = H.cfg.positivity ? 1 : 0;
     const size_t ncell = (size_t)A.nxu * A.nyu;
     // variant: 0 auto (fused), -1 split with the VALU cell kernel, 2..4 split with the
     // P=3 MFMA cell kernel at that occupancy, 10+R fused with R-row segments
@@ -2447,7 +2545,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             ProfScope pd(H, "dg_tvb_defer");
             hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.means, H.src,
                                H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.dcount, stage, H.dcells,
-                               H.tvb_linear);
+                               H.tvb_linear, A.zs_ext != 0);
         } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper
@@ -2456,14 +2554,15 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256),
                                0, H.stream,
                                Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
-                               H.g.gamma, H.scal, RA, nscan, H.tvb_linear);
+                               H.g.gamma, H.scal, RA, nscan, H.tvb_linear, A.zs_ext != 0);
         } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.tvb_linear);
+                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.tvb_linear,
+                               A.zs_ext != 0);
         else
             hipLaunchKernelGGL((dg_tvb_mfma_kernel<P>), dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, H.stream,
                                Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, A.zsok,
-                               H.scal, H.tvb_linear);
+                               H.scal, H.tvb_linear, A.zs_ext != 0);
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
         {
@@ -2491,16 +2590,31 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             hipLaunchKernelGGL((dg_learned_kernel<P, true>), dim3((unsigned)(want < cap ? want : cap)), dim3(128), 0,
                                H.stream, Qout, H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2,
                                H.g.gamma, H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist,
-                               H.scal);
+                               H.scal, A.zs_ext != 0);
         } else {
             hipLaunchKernelGGL((dg_learned_kernel<P, false>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout,
                                H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma,
-                               H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist, H.scal);
+                               H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist, H.scal,
+                               A.zs_ext != 0);
         }
     }
 }
 
 }  // namespace
+
+MfmaTabsX make_mfma_tables_ext(int P)
+{
+    MfmaTabsX X{};
+    double xs[20], ys[20];
+    zs_ext_points(P, GLP_H[P + 1], xs, ys);
+    for (int h = 0; h < 2; ++h)
+        for (int s = 0; s < 4; ++s)
+            for (int l = 0; l < 64; ++l) {
+                const int i = l % 16, py = l / 16, e = 16 * h + i < 20 ? 16 * h + i : 0;
+                X.zxA[h][s][l] = (s <= P && py <= P) ? legP_h(s, xs[e]) * legP_h(py, ys[e]) : 0.0;
+            }
+    return X;
+}
 
 // Uploads the basis and quadrature tables to the current device (once per
 // device); returns the first HIP error.
@@ -2520,6 +2634,9 @@ hipError_t dg_upload_tables(int device)
     if (e != hipSuccess) return e;
     const MfmaTabs4 F4[3] = {make_mfma_tables4(1), make_mfma_tables4(2), make_mfma_tables4(3)};
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_mf4), F4, sizeof(F4), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    const MfmaTabsX FX[3] = {make_mfma_tables_ext(1), make_mfma_tables_ext(2), make_mfma_tables_ext(3)};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_mfx), FX, sizeof(FX), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     if (track) g_tables_ready[device] = true;
     return hipSuccess;
