@@ -112,3 +112,26 @@ def test_training_gains_over_untrained():
     assert after["recall"] > 0.8
     ev = learned.evaluate(net, seed=999, n=96, steps=20)
     assert ev["finite"] and ev["l1_rho_vs_tvb"] < 1e-2
+
+
+def test_shipped_network_against_plain_tvb():
+    """What the shipped network gains over the method without it, on held-out
+    seeds of the C4 draws at 512^2 (60 steps), against plain TVB from the same
+    state and dt.  It gains nothing measurable: its decisions on audited
+    candidates beat a constant answer (balanced accuracy above 0.5, recall
+    above 0.7), but its density departs from plain TVB by about 4e-4 (relative
+    L1; bounded here at 1e-3, above the 1e-4 the chosen forms aim at) and its
+    run is slower, since the TVB test reads four neighbour means per candidate
+    and the network spends 31 kflop.  The rates are printed, not asserted."""
+    say("shipped network vs plain TVB, seeds 60..62 at 512^2")
+    net = learned.load()
+    rows = [learned.evaluate(net, seed=s, n=512, steps=60) for s in (60, 61, 62)]
+    for r in rows:
+        print(f"[learned vs tvb] seed {r['seed']}: L1(rho) {r['l1_rho_vs_tvb']:.2e}, "
+              f"rate {r['learned_cell_stage_per_s'] / r['tvb_cell_stage_per_s']:.2f}x of TVB, "
+              f"balanced accuracy {r['balanced_accuracy']:.2f}, recall {r['recall']:.2f}", flush=True)
+        assert r["finite"]
+        assert r["l1_rho_vs_tvb"] <= 1e-3
+        assert r["balanced_accuracy"] > 0.5
+        assert r["recall"] > 0.7
+    net.close()

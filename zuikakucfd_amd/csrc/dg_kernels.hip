@@ -1694,103 +1694,20 @@ __device__ __forceinline__ bool tvb_test(const double C[4][(P + 1) * (P + 1)], c
     return any;
 }
 
-// TVB limiter then positivity scaling in the matrix-core layout of the fused
-// kernel: a wave owns 16 consecutive cells of the update range, lane (g, col)
-// holds modes (px = v, py = g) of cell col.  The TVB test of a cell runs in its
-// lane of group 0 (mean and x-linear mode there, the y-linear mode shuffled in
-// from group 1, and at P >= 3 the (0,q) modes from group q for the face-average
-// deviations) and its outcome is broadcast down the column; the positivity
-// scaling evaluates rho and then all variables at the 4x4 points with MFMA
-// products and takes the minimum theta across the column's lanes.  Same
-// results as dg_tvb_kernel up to the summation order of the point values.
-//
-// With zsok (written by the fused stage kernel) the kernel first loads only
-// what the TVB test reads (means and the modes of degree 0 on one axis); a wave whose cells are
-// all unlimited and certified positive by zsok ends there, and only the other
-// waves load every mode for the limiting and the positivity scaling.
-// The TVB + positivity work of the 16 cells of one wave (cell k of column
-// lane col; live is false on padding lanes, which compute on a valid cell and
-// store nothing).  The whole wave must call it.
+// Positivity scaling (zs_cell) of the 16 cells of a wave held in the
+// matrix-core layout of tvb_mfma_cells (lane (g, col): modes (px = v, py = g)
+// of cell col): rho and then every variable at the 4x4 points (and, with ext,
+// at the positivity option's extra points) by MFMA products, theta reduced
+// across each column's lanes.  Returns whether this lane's cell changed.  The
+// whole wave must call it.
 template <int P>
-__device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const double *__restrict__ Mn,
-                                               const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
-                                               uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
-                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live,
-                                               bool lin, bool ext = false)
+__device__ __forceinline__ bool zs_mfma_apply(double C[4][4], double gamma, bool ext)
 {
-    constexpr int NQ = P + 1, NM = NQ * NQ;
+    constexpr int NQ = P + 1;
     const MfmaTabsF &MT = c_mff[P - 1];
     const MfmaTabsX &MX = c_mfx[P - 1];
     constexpr int NXCH = (4 * NQ + 4 + 15) / 16;  // chains of 16 extra points (positivity option)
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-    const bool slot = g <= P;
-    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
-    double C[4][4];
-    // Modes the TVB test reads: (0,0), (1,0) in group 0 and (0,1) in group 1;
-    // at P >= 3 every mode (q,0) in group 0 and every mode (0,q) in group q.
-    auto tvb_reads = [g](int v) {
-        return P >= 3 ? ((g == 0 && v <= P) || (v == 0 && g <= P)) : ((g == 0 && v <= 1) || (g == 1 && v == 0));
-    };
-    // phase 1: the modes the TVB test reads
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            C[c][v] = tvb_reads(v) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
-
-    // ---- TVB (cells with bctype 0) ----
-    int chg = 0;  // bit c: variable c limited
-    double axl[4], ayl[4];
-    const bool tvb_cell = bct[k] == 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const double ay_in = __shfl(C[c][0], 16 + col);   // mode (0, 1) of this column's cell
-        const double ub = C[c][0], ax = C[c][1];          // in group 0: modes (0, 0) and (1, 0)
-        double xp = ax, xm = ax, yp = ay_in, ym = ay_in;
-        if constexpr (P >= 3) {
-            // face-average deviations: modes (q, 0) in this lane, (0, q) from group q
-            const double qx[3] = {C[c][1], C[c][2], C[c][3]};
-            const double qy[3] = {ay_in, __shfl(C[c][0], 32 + col), __shfl(C[c][0], 48 + col)};
-            face_devs<3>(qx, xp, xm);
-            face_devs<3>(qy, yp, ym);
-        }
-        double a_x = ax, a_y = ay_in;
-        if (g == 0 && tvb_cell) {
-            if (tvb_axes(ub, ax, ay_in, P >= 3 && !lin, xp, xm, yp, ym, Mn, src, np, k, nx, c, Mh2, a_x, a_y))
-                chg |= 1 << c;
-        }
-        axl[c] = __shfl(a_x, col);
-        ayl[c] = __shfl(a_y, col);
-    }
-    chg = __shfl(chg, col);
-    {
-        const bool sure = zsok && (zsok[k] & 1);  // positivity scaling certified to be a no-op
-        const bool need = live && (chg != 0 || !sure);
-        if (!__any(need)) {
-            if (live && g == 0) flags[k] = 0;
-            return;
-        }
-    }
-    // phase 2: every mode
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (!tvb_reads(v) && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
-        }
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        if (chg & (1 << c))
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double nv = 0.0;
-                if (v == 0 && g == 0) nv = C[c][0];
-                else if (v == 1 && g == 0) nv = axl[c];
-                else if (v == 0 && g == 1) nv = ayl[c];
-                C[c][v] = nv;
-            }
-
-    // ---- positivity scaling (every cell of the range), as zs_cell ----
     constexpr double RHO_MIN = 1e-5, P_MIN = 1e-5, EPS = 1e-5;
     bool changed = false;
     double m0[4];
@@ -1887,6 +1804,129 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
                     if (v != 0 || g != 0) C[c][v] *= th;
         }
     }
+    return changed;
+}
+
+// Positivity scaling alone of 16 cells (cell k of column lane col; live is
+// false on padding lanes, which compute on a valid cell and store nothing), in
+// the matrix-core layout.  The whole wave must call it.
+template <int P>
+__device__ __forceinline__ void zs_mfma_cells(double *__restrict__ Q, size_t np, double gamma, size_t k, bool live,
+                                              bool ext)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const bool slot = g <= P;
+    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    double C[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            C[c][v] = (v <= P && slot) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+    const bool changed = zs_mfma_apply<P>(C, gamma, ext);
+    if (live && changed && slot)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] = C[c][v];
+}
+
+// TVB limiter then positivity scaling in the matrix-core layout of the fused
+// kernel: a wave owns 16 consecutive cells of the update range, lane (g, col)
+// holds modes (px = v, py = g) of cell col.  The TVB test of a cell runs in its
+// lane of group 0 (mean and x-linear mode there, the y-linear mode shuffled in
+// from group 1, and at P >= 3 the (0,q) modes from group q for the face-average
+// deviations) and its outcome is broadcast down the column; the positivity
+// scaling evaluates rho and then all variables at the 4x4 points with MFMA
+// products and takes the minimum theta across the column's lanes.  Same
+// results as dg_tvb_kernel up to the summation order of the point values.
+//
+// With zsok (written by the fused stage kernel) the kernel first loads only
+// what the TVB test reads (means and the modes of degree 0 on one axis); a wave whose cells are
+// all unlimited and certified positive by zsok ends there, and only the other
+// waves load every mode for the limiting and the positivity scaling.
+// The TVB + positivity work of the 16 cells of one wave (cell k of column
+// lane col; live is false on padding lanes, which compute on a valid cell and
+// store nothing).  The whole wave must call it.
+template <int P>
+__device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const double *__restrict__ Mn,
+                                               const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
+                                               uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
+                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live,
+                                               bool lin, bool ext = false)
+{
+    constexpr int NQ = P + 1, NM = NQ * NQ;
+    const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    const bool slot = g <= P;
+    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    double C[4][4];
+    // Modes the TVB test reads: (0,0), (1,0) in group 0 and (0,1) in group 1;
+    // at P >= 3 every mode (q,0) in group 0 and every mode (0,q) in group q.
+    auto tvb_reads = [g](int v) {
+        return P >= 3 ? ((g == 0 && v <= P) || (v == 0 && g <= P)) : ((g == 0 && v <= 1) || (g == 1 && v == 0));
+    };
+    // phase 1: the modes the TVB test reads
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            C[c][v] = tvb_reads(v) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+
+    // ---- TVB (cells with bctype 0) ----
+    int chg = 0;  // bit c: variable c limited
+    double axl[4], ayl[4];
+    const bool tvb_cell = bct[k] == 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double ay_in = __shfl(C[c][0], 16 + col);   // mode (0, 1) of this column's cell
+        const double ub = C[c][0], ax = C[c][1];          // in group 0: modes (0, 0) and (1, 0)
+        double xp = ax, xm = ax, yp = ay_in, ym = ay_in;
+        if constexpr (P >= 3) {
+            // face-average deviations: modes (q, 0) in this lane, (0, q) from group q
+            const double qx[3] = {C[c][1], C[c][2], C[c][3]};
+            const double qy[3] = {ay_in, __shfl(C[c][0], 32 + col), __shfl(C[c][0], 48 + col)};
+            face_devs<3>(qx, xp, xm);
+            face_devs<3>(qy, yp, ym);
+        }
+        double a_x = ax, a_y = ay_in;
+        if (g == 0 && tvb_cell) {
+            if (tvb_axes(ub, ax, ay_in, P >= 3 && !lin, xp, xm, yp, ym, Mn, src, np, k, nx, c, Mh2, a_x, a_y))
+                chg |= 1 << c;
+        }
+        axl[c] = __shfl(a_x, col);
+        ayl[c] = __shfl(a_y, col);
+    }
+    chg = __shfl(chg, col);
+    {
+        const bool sure = zsok && (zsok[k] & 1);  // positivity scaling certified to be a no-op
+        const bool need = live && (chg != 0 || !sure);
+        if (!__any(need)) {
+            if (live && g == 0) flags[k] = 0;
+            return;
+        }
+    }
+    // phase 2: every mode
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (!tvb_reads(v) && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+        }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (chg & (1 << c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double nv = 0.0;
+                if (v == 0 && g == 0) nv = C[c][0];
+                else if (v == 1 && g == 0) nv = axl[c];
+                else if (v == 0 && g == 1) nv = ayl[c];
+                C[c][v] = nv;
+            }
+
+    // ---- positivity scaling (every cell of the range), as zs_cell ----
+    const bool changed = zs_mfma_apply<P>(C, gamma, ext);
     if (!live) return;
     if (g == 0) flags[k] = (uint8_t)(chg != 0);
     if ((chg != 0 || changed) && slot)
@@ -2020,20 +2060,20 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
 }
 
 // TVB limiter after the fused stage, one lane per cell.  A wave takes 64
-// consecutive cells of the update range and reads their marks.  A certified
-// cell (bit 0: its positivity bounds clear the floors) needs no positivity
-// scaling even when the TVB test limits it, because limiting keeps the means
-// and never raises the magnitude of a mode (zeroed modes; a TVB-modified
-// minmod of the linear modes), so the certificate's bounds only tighten.  Such
-// a cell that is also an updated candidate (bit 1) gets the TVB test in its own
-// lane, from the modes the test reads (coalesced across the wave), and a
-// limited variable gets its linear modes replaced and its other modes zeroed;
-// every other certified cell gets flag 0.  The few uncertified cells are
-// listed (dcount[stage], dcells; one atomic per wave) for dg_tvb_defer_kernel,
-// which gives them the matrix-core TVB + positivity routine; keeping that
-// routine out of this kernel keeps its registers low, so more waves hide the
-// latency of the scattered neighbour-mean loads.  Results equal the 16-cell
-// scan's, and the oracle's TVB-then-positivity pass.
+// consecutive cells of the update range and reads their marks.  An updated
+// candidate (bit 1) gets the TVB test in its own lane, from the modes the test
+// reads (coalesced across the wave), and a limited variable gets its linear
+// modes replaced and its other modes zeroed; every cell gets its flag.  The
+// positivity scaling follows (the oracle's order: TVB on every cell, then the
+// scaling): a certified cell (bit 0: its positivity bounds clear the floors)
+// needs none even when the TVB test limited it, because limiting keeps the
+// means and never raises the magnitude of a mode (zeroed modes; a
+// TVB-modified minmod of the linear modes), so the certificate's bounds only
+// tighten.  The few uncertified cells are listed (dcount[stage], dcells; one
+// atomic per wave) for dg_tvb_defer_kernel, which applies the scaling to them
+// with the matrix-core routine; keeping that routine out of this kernel keeps
+// its registers low, so more waves hide the latency of the neighbour-mean
+// loads.  Results equal the 16-cell scan's, and the oracle's.
 template <int P>
 __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
@@ -2061,7 +2101,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
         k = interior_cell(t, nx, nxu);
         const unsigned z = zsok[k];
         defer = !(z & 1);
-        if (!defer) {
+        {
             bool lim = false;
             if ((z & 2) && upd[k]) {
                 // every load first (the stores below would otherwise order them):
@@ -2121,18 +2161,15 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
     if (defer) dcells[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)k;
 }
 
-// The uncertified cells the lane kernel listed (dcount[slot] of them): the
-// matrix-core TVB + positivity routine, 16 cells per wave per pass, over a
-// grid-strided range.  Block 0 clears the next stage's count (the counts
+// The uncertified cells the lane kernel listed (dcount[slot] of them), already
+// limited: the matrix-core positivity scaling, 16 cells per wave per pass, over
+// a grid-strided range.  Block 0 clears the next stage's count (the counts
 // rotate over the three stages, so a captured step needs no memset).
 template <int P>
-__global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
-                                                           const int32_t *__restrict__ src,
-                                                           const int32_t *__restrict__ bct,
-                                                           uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
-                                                           double gamma, const StepScalars *__restrict__ S,
+__global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ Q, size_t np, double gamma,
+                                                           const StepScalars *__restrict__ S,
                                                            unsigned *__restrict__ dcount, int slot,
-                                                           const int32_t *__restrict__ dcells, bool lin, bool ext)
+                                                           const int32_t *__restrict__ dcells, bool ext)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) dcount[(slot + 1) % 3] = 0u;
     if (S->done) return;  // uniform over the grid
@@ -2141,8 +2178,7 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
     const unsigned waves = gridDim.x * (blockDim.x >> 6);
     for (unsigned g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u; g0 < n; g0 += waves * 16u) {
         const unsigned i = g0 + col < n ? g0 + col : n - 1;  // padding lanes repeat the last cell, store nothing
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, (size_t)dcells[i], g0 + col < n,
-                          lin, ext);
+        zs_mfma_cells<P>(Q, np, gamma, (size_t)dcells[i], g0 + col < n, ext);
     }
 }
 
@@ -2543,9 +2579,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             }
             const unsigned nblk = (unsigned)(H.ncu > 0 ? H.ncu : 256);
             ProfScope pd(H, "dg_tvb_defer");
-            hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.means, H.src,
-                               H.bct, H.flags, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.dcount, stage, H.dcells,
-                               H.tvb_linear, A.zs_ext != 0);
+            hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.g.np,
+                               H.g.gamma, H.scal, H.dcount, stage, H.dcells, A.zs_ext != 0);
         } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
             // (the passes are long); at P <= 2 the wider scan is cheaper

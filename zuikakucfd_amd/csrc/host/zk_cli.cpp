@@ -155,6 +155,7 @@ struct Opts {
     bool slip = false;
     std::string weights;
     int ranks = 1;  // per-rank output files as a `mpirun -np ranks` run would write
+    int every = 100;  // output interval in steps (the reference's 100); 0: the last step only
     float band_lo = 0.2f, band_hi = 0.8f, audit = 0.02f;
 };
 
@@ -198,6 +199,7 @@ bool parse_opts(int argc, char **argv, Opts &o)
                 o.band_hi = std::stof(argv[++a]);
             } else if (k == "--audit" && val(v)) o.audit = std::stof(v);
             else if (k == "--ranks" && val(v)) o.ranks = std::max(1, std::stoi(v));
+            else if (k == "--output-every" && val(v)) o.every = std::max(0, std::stoi(v));
             else if (k == "--dump") o.dump = true;
             else if (k == "--quiet") o.quiet = true;
             else if (k == "--positivity") o.positivity = true;
@@ -373,7 +375,9 @@ int main(int argc, char **argv)
     const double hh = in.da;
     double t = 0.0, dt = 0.0, compute_ms = 0.0, io_ms = 0.0;
     int step = 0;
-    const int output_interval = 100;
+    // the reference writes every 100 steps (src_DG/solver.cpp:267); --output-every
+    // changes that, 0 keeping the last step only
+    const int output_interval = o.every > 0 ? o.every : (1 << 30);
 
 #ifdef ZK_CLI_WENO
     std::cout << "================================\n2D Euler WENO5 solver (MI355X)\n"

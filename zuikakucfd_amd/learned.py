@@ -23,8 +23,8 @@ WEIGHTS = DATA / "troubled_cell_p2.npz"
 CFL_P2 = 0.9 / (2.0 * (2 * 2 + 1))
 
 
-def dg_solver(m, limiter: int, M: float = 50.0) -> zk.Solver:
-    s = zk.Solver(method=zk.METHOD_DG, P=2, limiter=limiter, tvb_M=M)
+def dg_solver(m, limiter: int, M: float = 50.0, positivity: bool = False) -> zk.Solver:
+    s = zk.Solver(method=zk.METHOD_DG, P=2, limiter=limiter, tvb_M=M, positivity=positivity)
     s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
     return s
 
@@ -67,22 +67,45 @@ def load(path=WEIGHTS, device: int = 0) -> zk.Mlp:
     return mlp
 
 
+def timed_run(s: zk.Solver, dt: float, steps: int, chunk: int = 500) -> tuple[float, int | None]:
+    """Advance `steps` steps in timed chunks (HIP events on the handle's
+    stream); (milliseconds, None), or (milliseconds so far, the step count of
+    the check that found a non-finite state)."""
+    ms, done = 0.0, 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        try:
+            ms += s.step_timed(dt, k)
+        except zk.ZkNumericError:
+            return ms, done + k
+        done += k
+    return ms, None
+
+
 def evaluate(mlp: zk.Mlp, seed: int, n: int, steps: int, lo: float = 0.2, hi: float = 0.8,
-             audit: float = 0.02) -> dict:
+             audit: float = 0.02, positivity: bool = False) -> dict:
     """Plain TVB and learned runs of one problem from the same state with the
     same fixed dt: rates, density L1 difference, the network's miss and
-    false-flag rates on audited cells and the share of cells in the band."""
+    false-flag rates on audited cells and the share of cells in the band.
+    positivity: both runs with the DG positivity option."""
     m = meshio.random_riemann(int(seed), n)
-    plain = dg_solver(m, zk.LIMITER_TVB)
+    plain = dg_solver(m, zk.LIMITER_TVB, positivity=positivity)
     dt = CFL_P2 * m.da / max(plain.max_speed(), 1e-7)
-    learned = dg_solver(m, zk.LIMITER_LEARNED)
+    learned = dg_solver(m, zk.LIMITER_LEARNED, positivity=positivity)
     learned.attach_mlp(mlp, lo, hi, audit)
     ncell = plain.updated_cells()
-    out = {"seed": int(seed), "n": n, "steps": steps, "dt": dt, "lo": lo, "hi": hi, "audit": audit}
+    out = {"seed": int(seed), "n": n, "steps": steps, "dt": dt, "lo": lo, "hi": hi, "audit": audit,
+           "positivity": positivity}
     for name, s in (("tvb", plain), ("learned", learned)):
         s.step(dt, 2)
-        ms = s.step_timed(dt, steps)  # HIP events on the handle's stream
-        out[f"{name}_cell_stage_per_s"] = 3 * ncell * steps / (ms / 1e3)
+        ms, fail = timed_run(s, dt, steps)
+        out[f"{name}_cell_stage_per_s"] = 3 * ncell * steps / (ms / 1e3) if fail is None else None
+        out[f"{name}_nonfinite_step"] = fail
+    if out["tvb_nonfinite_step"] or out["learned_nonfinite_step"]:
+        out["finite"] = False
+        plain.close()
+        learned.close()
+        return out
     Up, Ul = plain.get_conserved(), learned.get_conserved()
     mask = np.zeros_like(m.bctype, dtype=bool)
     mask[3:-3, 3:-3] = True
