@@ -106,13 +106,22 @@ def measured_traffic(workload: str, kernel: str):
 
 
 def _cpu_model() -> str:
+    """The first processor's model name as /proc/cpuinfo reports it (a
+    virtual machine may shorten it), with its family/model numbers and widest
+    vector extension, which identify the part when the name does not."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if not line.strip():
+                break
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    flags = info.get("flags", "").split()
+    simd = "avx512f" if "avx512f" in flags else "avx2" if "avx2" in flags else "sse"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, "
+            f"model {info.get('model', '?')}, {simd})")
 
 
 def _native_oracle() -> tuple[str | None, str]:

@@ -149,6 +149,24 @@ def agreement(mlp: zk.Mlp, seed: int, n: int, steps: int) -> dict:
     return {"audited": st["audited"], **rates(st)}
 
 
+def c4_problem(mlp: zk.Mlp, seed: int, n: int = 2048, steps: int = 2000) -> dict:
+    """One seeded four-quadrant DG P=2 problem run three ways for `steps`
+    fixed-dt steps at nu = CFL_P2: the reference scheme (TVB alone), then TVB
+    and the learned indicator, both with the DG positivity option (evaluate).
+    Non-finite states are found by the failure check every 500 steps."""
+    m = meshio.random_riemann(int(seed), n)
+    s = dg_solver(m, zk.LIMITER_TVB)
+    dt = CFL_P2 * m.da / max(s.max_speed(), 1e-7)
+    s.step(dt, 2)
+    ms, fail = timed_run(s, dt, steps)
+    ncell = s.updated_cells()
+    s.close()
+    out = evaluate(mlp, seed, n, steps, positivity=True)
+    out["plain_tvb_nonfinite_step"] = fail
+    out["plain_tvb_cell_stage_per_s"] = 3 * ncell * steps / (ms / 1e3) if fail is None else None
+    return out
+
+
 def build_default(seeds=range(0, 48), n: int = 96, steps: int = 60, epochs: int = 40, capacity: int = 400000,
                   path=WEIGHTS) -> dict:
     """Harvest, train and save the network shipped with the package."""
@@ -175,7 +193,17 @@ def main(argv=None) -> int:
     ap.add_argument("--capacity", type=int, default=1_500_000)
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[0, 1])
     ap.add_argument("--eval-steps", type=int, default=20)
+    ap.add_argument("--c4", type=int, nargs=2, metavar=("A", "B"),
+                    help="instead of training: run c4_problem on seeds [A, B) with the shipped network, "
+                         "one JSON line per seed (--n, --c4-steps)")
+    ap.add_argument("--c4-steps", type=int, default=2000)
     a = ap.parse_args(argv)
+    if a.c4:
+        mlp = load()
+        for seed in range(*a.c4):
+            print(json.dumps(c4_problem(mlp, seed, a.n, a.c4_steps)), flush=True)
+        mlp.close()
+        return 0
     mlp = zk.Mlp(seed=1234, capacity=a.capacity)
     t0 = time.perf_counter()
     for seed in range(*a.seeds):
