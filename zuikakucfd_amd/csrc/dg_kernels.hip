@@ -264,7 +264,17 @@ struct DgArgs {
     double ca, cb;
     int stage, flux, limiter;
     int zs_ext;       // positivity option: the scaling also at the face points and corners
+    int row16;        // rows may be staged with 16-byte loads (nx even: every row segment 16-byte aligned)
 };
+
+// Plane `pl` of a plane-major array, `stride` bytes apart.  With a 32-bit
+// stride (arrays below 4 GiB) the offset is one 32-bit scalar multiply.
+template <typename Ofs, typename T>
+__device__ __forceinline__ T *plane_at(T *base, int pl, Ofs stride)
+{
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + (size_t)((Ofs)pl * stride));
+}
 
 template <int P, bool X>
 __device__ __forceinline__ void load_trace(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
@@ -1160,9 +1170,15 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // the sum-factorised form: values are contracted over py (the lane group) by
 // the matrix core, four cells per block, and over px (the register index) by
 // VALU multiply-adds.  Same lane layout, same results up to summation order.
-template <int P, int MINW, bool WG, int MODE, bool F4>
+//
+// SK: 1 = the first RK stage, 2 = a later one, fixed at compile time so that
+// the stage combination carries no selects (0 reads A.stage).  O32: plane
+// offsets in 32 bits (state arrays below 4 GiB), which shortens the scalar
+// address arithmetic of the 3 x 4 (P+1)^2 plane accesses per row.
+template <int P, int MINW, bool WG, int MODE, bool F4, int SK = 0, bool O32 = false>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
+    using Ofs = typename std::conditional<O32, uint32_t, size_t>::type;
     constexpr int NQ = P + 1, NM = NQ * NQ;
     constexpr bool DMA = MODE >= 2, DB = MODE == 3;
     constexpr int NB = DB ? 2 : 1, NS = DB ? 4 : 2;  // row buffers, ghost-map row slots
@@ -1178,6 +1194,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     __shared__ double qbuf[DMA ? NB : 1][DMA ? 4 * NM : 1][DMA ? 64 : 1];  // rows: plane (c, mode) x 64 columns
     __shared__ int32_t srcbuf[DMA ? NS : 1][DMA ? 64 : 1];               // ghost-map entries of row r: slot r % NS
     if (A.S->done) return;
+    const bool first = SK == 1 ? true : SK == 2 ? false : A.stage == 0;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
     const int wv = threadIdx.x >> 6;
@@ -1255,18 +1272,34 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
         return A.src[(size_t)row * A.nx + jj];
     };
-    auto dma_row = [&](int row, int32_t e0, int32_t e1, size_t npl) {
+    auto dma_row = [&](int row, int32_t e0, int32_t e1, Ofs npb) {
         auto qb = qbuf[DB ? (row & 1) : 0];
         const uint32_t b = (uint32_t)(lane & 1) * 4u;
         const uint32_t o0 = (uint32_t)(e0 >> kSrcShift) * 8u + b, o1 = (uint32_t)(e1 >> kSrcShift) * 8u + b;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             const int pl = wvu * NM + m;
-            const char *plane = reinterpret_cast<const char *>(A.Qin + (size_t)pl * npl);
+            const char *plane = reinterpret_cast<const char *>(plane_at(A.Qin, pl, npb));
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o0),
                                              (void __attribute__((address_space(3))) *)&qb[pl][0], 4, 0, 0);
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o1),
                                              (void __attribute__((address_space(3))) *)&qb[pl][32], 4, 0, 0);
+        }
+    };
+    // A row whose 64 columns are their own sources (no ghost, mirror or
+    // clamped column) is copied as it lies, two planes per 16-byte load: lane
+    // l moves columns 2 (l % 32) and 2 (l % 32) + 1 of plane pl + l / 32.
+    // Every such segment ends inside its row (column jrun0 + 63 < nx).
+    auto dma_row16 = [&](int row, Ofs npb) {
+        auto qb = qbuf[DB ? (row & 1) : 0];
+        const Ofs o = (Ofs)(((size_t)row * A.nx + jrun0) * 8 + (size_t)(lane & 31) * 16) + ((lane >> 5) ? npb : (Ofs)0);
+#pragma unroll
+        for (int m = 0; m < NM; m += 2) {
+            const char *plane = reinterpret_cast<const char *>(plane_at(A.Qin, wvu * NM + m, npb));
+            if (m + 1 < NM || lane < 32)  // an odd plane count ends with one plane (lanes 0-31)
+                __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o),
+                                                 (void __attribute__((address_space(3))) *)&qb[wvu * NM + m][0],
+                                                 16, 0, 0);
         }
     };
     auto dma_src = [&](int row) {
@@ -1280,7 +1313,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             (void __attribute__((address_space(3))) *)&srcbuf[row % NS][0], 4, 0, 0);
     };
     if (DMA) {
-        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), np);
+        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), (Ofs)(np * 8));
         if (wvu == 0) {
             dma_src(r0);
             if (r0 + 1 <= r1) dma_src(r0 + 1);
@@ -1327,8 +1360,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         asm volatile("" : "+v"(ln));
         // likewise the plane stride: the plane addresses are formed per row
         // (scalar arithmetic) instead of being held across the loop
-        size_t npr = np;
-        asm volatile("" : "+s"(npr));
+        Ofs npb = (Ofs)(np * 8);
+        asm volatile("" : "+s"(npb));
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
         // With two row buffers, row r stays in LDS for the whole iteration, so
@@ -1384,7 +1417,15 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             if (!DB) __syncthreads();
             if (inner) {
                 const int32_t *sn = srcbuf[(r + 1) % NS];
-                dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)], npr);
+                // one column per lane: is row r + 1 its own source throughout?
+                const int xs = ln & 63;
+                const int32_t es = sn[xs];
+                const bool own = jrun0 + xs < A.nx && (es & 3) == 0 &&
+                                 (es >> kSrcShift) == (int32_t)((r + 1) * A.nx + jrun0 + xs);
+                if (A.row16 && __ballot(own) == ~0ull)
+                    dma_row16(r + 1, npb);
+                else
+                    dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)], npb);
                 if (wvu == 0 && r + 2 <= r1) dma_src(r + 2);
             }
         } else {
@@ -1455,7 +1496,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v <= P; ++v) {
                     const double out = base[c][v] - fs[v] * gN[c][v];
-                    (A.Qout + (size_t)(c * NM + v * NQ) * npr)[gofs + (uint32_t)kp] = out;
+                    plane_at(A.Qout, c * NM + v * NQ, npb)[gofs + (uint32_t)kp] = out;
                     if (v == 0 && g == 0) {
                         mean0[c] = out;
                         if (A.means) A.means[(size_t)c * np + kp] = out;
@@ -1508,11 +1549,11 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // stage U^n is the row just read (an updated cell's ghost-map entry is
         // itself, unsigned), so only the cells that are not updated read it.
         double qn[4][4];
-        if (store_col && slot && !(A.stage == 0 && upd))
+        if (store_col && slot && !(first && upd))
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v <= P; ++v) qn[c][v] = (A.Qn + (size_t)(c * NM + v * NQ) * npr)[gofs + (uint32_t)k];
+                for (int v = 0; v <= P; ++v) qn[c][v] = plane_at(A.Qn, c * NM + v * NQ, npb)[gofs + (uint32_t)k];
         // rhs of row r, all terms but its S face: N face, x faces, then the volume
         // projection accumulated on top by the MFMA
         d4 acc[4];
@@ -1637,7 +1678,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             }
         }
         // combination of row r up to its S-face term
-        const double cdt = (A.stage == 0) ? dt : A.cb * dt;
+        const double cdt = first ? dt : A.cb * dt;
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
         if (DB && P != 2) reread(Q);
@@ -1645,8 +1686,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // (fs = cb dt fdu, 0 for cells not updated); the first stage is
         // Q + fs acc for updated cells and qn (not read otherwise) for the rest.
         // At P = 3 the two stage branches cost registers (spills): one
-        // expression there.
-        if constexpr (P == 3) {
+        // expression there, unless the stage is fixed at compile time.
+        if constexpr (P == 3 && SK == 0) {
             if (store_col && slot) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
@@ -1658,7 +1699,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     }
             }
         } else if (store_col && slot) {
-            if (A.stage == 0) {
+            if (first) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -2476,7 +2517,13 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const int width = wg ? 62 : 14;                 // updated columns per wave (per workgroup for wg)
         const int nrun = (A.nxu + width - 1) / width;
         // the kernel instance this stage launches
-        const void *kern = (wg && H.dma == 3 && H.mf4) ? (const void *)dg_fused_kernel<P, 2, true, 3, true>
+        // the default form with the stage and 32-bit plane offsets fixed at
+        // compile time, when the state arrays are below 4 GiB
+        const bool fast = wg && H.dma == 3 && H.mf4 && (size_t)4 * (P + 1) * (P + 1) * H.g.np * 8 <= 0xFFFFFFFFull;
+        A.row16 = (A.nx % 2 == 0) ? 1 : 0;
+        const void *kern = (fast && stage == 0)       ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true>
+                           : fast                     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true>
+                           : (wg && H.dma == 3 && H.mf4) ? (const void *)dg_fused_kernel<P, 2, true, 3, true>
                            : (wg && H.dma == 3)         ? (const void *)dg_fused_kernel<P, 2, true, 3, false>
                            : (wg && H.dma == 2)         ? (const void *)dg_fused_kernel<P, 2, true, 2, false>
                            : (wg && H.dma == 1)         ? (const void *)dg_fused_kernel<P, 2, true, 1, false>
@@ -2486,8 +2533,9 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             // Whole rounds of the device: as many workgroups (or single waves)
             // as the CUs hold at the kernel's occupancy.  The fewest rounds with
             // R <= 64 keep halo rows few and leave no part-filled last round.
-            static std::atomic<int> occ_cache[6];  // per kernel instance of this P (0 = not known yet)
-            const int slot_i = !wg ? 5 : H.dma == 3 ? (H.mf4 ? 0 : 1) : H.dma == 2 ? 2 : H.dma == 1 ? 3 : 4;
+            static std::atomic<int> occ_cache[8];  // per kernel instance of this P (0 = not known yet)
+            const int slot_i = fast ? (stage == 0 ? 6 : 7)
+                               : !wg ? 5 : H.dma == 3 ? (H.mf4 ? 0 : 1) : H.dma == 2 ? 2 : H.dma == 1 ? 3 : 4;
             int per_cu = occ_cache[slot_i].load();
             if (per_cu < 1) {
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)
