@@ -99,10 +99,16 @@ def measured_traffic(workload: str, kernel: str):
     except (OSError, KeyError, ValueError):
         return None
     hits = [v for k, v in d["kernels"].items() if k.startswith(kernel + "_") or k.startswith(kernel + "<")]
-    if len(hits) != 1:
+    if not hits:
         return None
-    return {"bytes": hits[0]["bytes"], "read_bytes": hits[0]["read_bytes"], "write_bytes": hits[0]["write_bytes"],
-            "source": "profiles/" + d["source"]}
+    # several instances of one kernel (e.g. the first RK stage and the later
+    # ones): the average per launch, weighted by their launch counts
+    wsum = sum(h.get("dispatches", 1) for h in hits)
+    avg = {key: sum(h[key] * h.get("dispatches", 1) for h in hits) / wsum
+           for key in ("bytes", "read_bytes", "write_bytes")}
+    avg["instances"] = len(hits)
+    avg["source"] = "profiles/" + d["source"]
+    return avg
 
 
 def _cpu_model() -> str:
@@ -409,6 +415,11 @@ def main() -> int:
     ap.add_argument("--learned-problems", type=int, default=4,
                     help="seeded problems of the DG P=2 learned-indicator configuration (of 64)")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--save-state", default=None, metavar="NPY",
+                    help="write the state reached at the timed window (the warm-up's end) to NPY")
+    ap.add_argument("--resume-state", default=None, metavar="NPY",
+                    help="start the timed window from a state written by --save-state for the same workload and "
+                         "warm-up, instead of advancing to it (short runs under a profiler)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB on the device (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -444,6 +455,10 @@ def main() -> int:
     # advance, untimed, to the timed window
     done = args.early_warmup + args.steps
     t_adv = time.perf_counter()
+    if args.resume_state:
+        s.set_state(np.load(args.resume_state))
+        done = max(done, args.warmup)
+        log(f"resumed at step {done} from {args.resume_state}")
     while done < args.warmup:
         k = min(1000, args.warmup - done)
         s.step(dt, k)
@@ -452,6 +467,8 @@ def main() -> int:
         if done % 4000 < 1000 or done == args.warmup:
             log(f"advanced to step {done} of {args.warmup} ({time.perf_counter() - t_adv:.1f} s)")
     advance_s = time.perf_counter() - t_adv
+    if args.save_state:
+        np.save(args.save_state, s.get_state())
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc else None  # the timed window gets the host to itself
     ms = s.step_timed(dt, args.steps)
     rate = args.steps * 3 * nupd / (ms / 1e3)
@@ -526,6 +543,9 @@ def main() -> int:
         "last_step": last_step,
         "boundary": boundary,
     }
+    if args.resume_state:
+        line["resumed"] = ("the timed window started from the state a --save-state run reached at step "
+                           f"{args.warmup} (the untimed advance skipped)")
     tr = measured_traffic(args.workload, dom)
     if tr:
         line["roofline"]["traffic"] = tr["bytes"]

@@ -451,15 +451,20 @@ zk_status zk_ensemble_step(zk_ensemble *E, int nsteps)
 {
     if (!E || !E->final || nsteps < 0) return ZK_ERR_ARG;
     ENS_HIP(hipSetDevice(E->cfg.device));
-    if (E->use_graph) {
+    if (E->use_graph && nsteps > 0) {
         if (!E->exec) {
+            // the first step runs as plain launches, so that its kernels are
+            // loaded and launched once before the capture records them
+            enqueue_ens_step(E);
+            ENS_HIP(hipGetLastError());
+            --nsteps;
             ENS_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
             enqueue_ens_step(E);
             ENS_HIP(hipStreamEndCapture(E->stream, &E->graph));
             ENS_HIP(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
         }
         for (int s = 0; s < nsteps; ++s) ENS_HIP(hipGraphLaunch(E->exec, E->stream));
-    } else {
+    } else if (!E->use_graph) {
         for (int s = 0; s < nsteps; ++s) enqueue_ens_step(E);
         ENS_HIP(hipGetLastError());
     }

@@ -324,6 +324,11 @@ zk_status enqueue_steps(Handle *H, int nsteps)
     const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
     if (!salted && H->use_graph && !H->prof && nsteps >= 2) {
         if (!H->step_exec) {
+            // the first step runs as plain launches: every kernel of the step
+            // is then loaded and launched once before any capture records it
+            st = enqueue_step(H);
+            if (st) return st;
+            --nsteps;
             ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
             st = enqueue_step(H);
             hipGraph_t g = nullptr;
@@ -726,24 +731,29 @@ zk_status zk_step_cfl(zk_handle *h, double cfl, double t_end, int nsteps, double
     // and dt = 0: its stage kernels return at once, and the host stops.
     const bool salted = H->cfg.method == ZK_METHOD_DG && (H->cfg.limiter == ZK_LIMITER_LEARNED || H->harvest);
     const bool graph = !salted && H->use_graph && !H->prof;
-    if (graph && !H->cfl_exec) {
-        ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
-        st = enqueue_cfl_step(H);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(H->stream, &g);
-        if (st) {
-            if (g) (void)hipGraphDestroy(g);
-            return st;
-        }
-        ZK_HIP(e);
-        H->cfl_graph = g;
-        ZK_HIP(hipGraphInstantiate(&H->cfl_exec, g, nullptr, nullptr, 0));
-    }
     for (int k = 0; k < nsteps; k += 16) {
         const int chunk = nsteps - k < 16 ? nsteps - k : 16;
         for (int c = 0; c < chunk; ++c) {
-            if (graph) ZK_HIP(hipGraphLaunch(H->cfl_exec, H->stream));
-            else if ((st = enqueue_cfl_step(H)) != ZK_OK) return st;
+            if (graph && H->cfl_exec) {
+                ZK_HIP(hipGraphLaunch(H->cfl_exec, H->stream));
+                continue;
+            }
+            if ((st = enqueue_cfl_step(H)) != ZK_OK) return st;
+            if (graph) {
+                // captured after the first step ran as plain launches, so
+                // every kernel of the step is loaded before a capture records it
+                ZK_HIP(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
+                st = enqueue_cfl_step(H);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(H->stream, &g);
+                if (st) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return st;
+                }
+                ZK_HIP(e);
+                H->cfl_graph = g;
+                ZK_HIP(hipGraphInstantiate(&H->cfl_exec, g, nullptr, nullptr, 0));
+            }
         }
         H->steps_enqueued += chunk;
         ZK_HIP(hipMemcpyAsync(&s, H->scal, sizeof(s), hipMemcpyDeviceToHost, H->stream));
