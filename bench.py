@@ -376,7 +376,22 @@ def path_learned(steps: int, problems: int, n: int = 2048) -> dict:
             **learned.rates(tot),
             "candidate_fraction": sum(r["candidate_fraction"] for r in rows) / len(rows),
             "band_fraction": tot["in_band"] / max(tot["candidates"], 1),
-            "finite": all(r["finite"] for r in rows)}
+            "finite": all(r["finite"] for r in rows),
+            "sweep_64x2000": recorded_c4_sweep()}
+
+
+def recorded_c4_sweep() -> dict | None:
+    """The summary of the recorded full C4 sweep (64 problems x 2,000 steps,
+    `python -m zuikakucfd_amd.learned --c4 0 64`, about half an hour of GPU
+    time, too long for this bench): finite counts, rates and the network's
+    rates over all audited cells."""
+    p = ROOT / "zuikakucfd_amd" / "data" / "c4_sweep_2048_2000.json"
+    try:
+        d = json.loads(p.read_text())["summary"]
+    except (OSError, KeyError, ValueError):
+        return None
+    d["recorded"] = f"zuikakucfd_amd/data/{p.name} (not measured by this run)"
+    return d
 
 
 def device_copy(nbytes: int) -> int:

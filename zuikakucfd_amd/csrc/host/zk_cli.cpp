@@ -8,12 +8,14 @@
 // libzk.so on the GPU.  Options after the three positional arguments select
 // what the reference fixes at compile time or does not offer:
 //   --flux hllc|hll          numerical flux
-//   --limiter persson|tvb|none|learned   (DG) limiter, default persson (the reference's)
+//   --limiter persson|tvb|tvb-linear|none|learned   (DG) limiter, default persson (the
+//                            reference's); tvb-linear: TVB without the P=3 face-average test
 //   --M <value>              TVB constant, default 50
 //   --precision fp64|fp32    (WENO) bulk arithmetic precision
 //   --cfl <nu>               (WENO) auto/time modes use dt = nu*h/lambda instead of the
 //                            reference's dt = lambda*h (src_WENO/solver.cpp:150,169)
-//   --positivity             (WENO) face positivity check with first-order fallback
+//   --positivity             WENO: face positivity check with first-order fallback;
+//                            DG: Zhang-Shu scaling also at the face points and corners
 //   --slip                   (WENO) bctype-1 cells are mirror slip walls (zk_set_slip_walls);
 //                            without it they are left alone, as the reference's WENO does
 //   --recon weno5|muscl-minmod|muscl-vanleer|muscl-superbee
@@ -26,6 +28,7 @@
 //   --audit <fraction>       (learned) random share of cells also checked by TVB, default 0.02
 //   --ranks <n>              write U<c>_<r>.dat for r = 0..n-1 as `mpirun -np n` would
 //                            (column slices with 3 exchanged ghost columns per inner side)
+//   --output-every <n>       output interval in steps, default 100; 0 = the last step only
 //   --dump                   also write result/<step>/state.f64 (all modes, full precision)
 //   --quiet                  no per-step progress lines
 #include <chrono>
