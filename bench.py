@@ -266,7 +266,8 @@ def path_ensemble(steps: int, ninst: int = 256) -> dict:
     """Config 5: 256 seeded four-quadrant problems, N uniform over
     {128,192,256,384,512}, per-instance CFL 0.5 dt, 200 steps.  Three rates
     separate the two gains: the plain FP64 per-instance loop (one handle per
-    problem, zk_step_cfl with its captured CFL step replayed), the FP64
+    problem, zk_step_cfl with its CFL step captured beforehand and replayed,
+    a host read of the step scalars every 16 steps), the FP64
     batched ensemble (batching alone) and the FP32-bulk batched ensemble
     (batching and precision; persistent launch, graph replay, autotuned
     tiles).  Density error of FP32 bulk against the FP64 loop per instance."""
@@ -313,8 +314,14 @@ def path_ensemble(steps: int, ninst: int = 256) -> dict:
         s = zk.Solver(method=zk.METHOD_WENO, positivity=True)
         s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
         solvers.append(s)
-    solvers[0].step_cfl(0.5, 1e30, 1)  # first capture outside the timing
-    solvers[0].load(ms[0].da, ms[0].gamma, ms[0].rho, ms[0].u, ms[0].v, ms[0].p, ms[0].bctype)
+    # every handle captures its CFL step outside the timing (one step, then
+    # the initial state back: zk_set_state keeps the buffers and the graph)
+    for s in solvers:
+        q0 = s.get_state()
+        s.step_cfl(0.5, 1e30, 1)
+        s.set_state(q0)
+    for s in solvers:
+        s.sync()
     t0 = time.perf_counter()
     for s in solvers:
         s.step_cfl(0.5, 1e30, steps)
@@ -558,6 +565,11 @@ def main() -> int:
         "last_step": last_step,
         "boundary": boundary,
     }
+    if args.workload.startswith("dg3_tvb"):
+        # what the counters of this kernel show (committed rocprofv3 summaries)
+        line["roofline"]["measured_limit"] = (
+            "SIMD issue, not HBM: the fused kernel's waves issue 26-34% of their cycles and wait to issue "
+            "45-52% (profiles/dg3_tvb_riemann1024_stall_pmc.txt); one wave per SIMD runs 0.89x as fast as two")
     if args.resume_state:
         line["resumed"] = ("the timed window started from the state a --save-state run reached at step "
                            f"{args.warmup} (the untimed advance skipped)")
