@@ -33,6 +33,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 (vector and matrix alike)
 METRIC = "cell-stage updates/s, chosen vs plain FP64 at matched L1 err; % HBM roofline"
 
 # t_end: the end time of the configuration (the default timed window sits at
@@ -107,6 +108,23 @@ def measured_traffic(workload: str, kernel: str):
     avg = {key: sum(h[key] * h.get("dispatches", 1) for h in hits) / wsum
            for key in ("bytes", "read_bytes", "write_bytes")}
     avg["instances"] = len(hits)
+    avg["source"] = "profiles/" + d["source"]
+    return avg
+
+
+def measured_flops(workload: str, kernel: str):
+    """Executed FP64 operations per launch of `kernel` for `workload` (VALU and
+    MFMA, from the committed rocprofv3 counter summary profiles/flops.json),
+    weighted over the kernel's instances by their launch counts, or None."""
+    try:
+        d = json.loads((ROOT / "profiles" / "flops.json").read_text())[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    hits = [v for k, v in d["kernels"].items() if k.startswith(kernel + "_") or k.startswith(kernel + "<")]
+    if not hits:
+        return None
+    wsum = sum(h.get("dispatches", 1) for h in hits)
+    avg = {key: sum(h[key] * h.get("dispatches", 1) for h in hits) / wsum for key in ("flops", "valu_flops", "mfma_flops")}
     avg["source"] = "profiles/" + d["source"]
     return avg
 
@@ -259,6 +277,13 @@ def path_jet(steps: int) -> dict:
     out["fp32_bulk_cell_stage_per_s"] = res["fp32_bulk"][0]
     out["speedup"] = res["fp32_bulk"][0] / res["fp64"][0]
     out["l1_rho_vs_fp64"] = _rel_l1(res["fp32_bulk"][1][0], res["fp64"][1][0])
+    out["fp64_hbm_frac"] = res["fp64"][0] * algo_bytes_per_cell_stage(1) / (HBM_PEAK_GBS * 1e9)
+    fl = measured_flops("weno_fp64_jet2048x1024", "fv_stage")
+    if fl:
+        # executed FP64 operations per cell-stage (counted) at the FP64 path's rate
+        per_cell = fl["flops"] / n
+        out["fp64_flops_per_cell_stage"] = per_cell
+        out["fp64_fp_frac"] = res["fp64"][0] * per_cell / (FP64_PEAK_TFS * 1e12)
     return out
 
 
@@ -573,6 +598,13 @@ def main() -> int:
     if args.resume_state:
         line["resumed"] = ("the timed window started from the state a --save-state run reached at step "
                            f"{args.warmup} (the untimed advance skipped)")
+    fl = measured_flops(args.workload, dom)
+    if fl and w["prec"] == "fp64":
+        # the second fraction: executed FP64 operations against the FP64 peak
+        tf = fl["flops"] / (avg_ms / 1e3) / 1e12
+        line["roofline"]["fp64"] = {"flops_per_launch": fl["flops"], "valu_flops": fl["valu_flops"],
+                                    "mfma_flops": fl["mfma_flops"], "achieved": tf, "peak": FP64_PEAK_TFS,
+                                    "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS, "source": fl["source"]}
     tr = measured_traffic(args.workload, dom)
     if tr:
         line["roofline"]["traffic"] = tr["bytes"]
