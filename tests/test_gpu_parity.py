@@ -226,18 +226,21 @@ def test_dg_random_mask(P, variant, nx):
 
 
 @pytest.mark.parametrize("P", [1, 2, 3])
-@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "3-16x16"])
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "3-16x16", "3-generic"])
 def test_dg_fused_memory_modes(P, mode, monkeypatch):
     """The fused kernel's memory pipelines and product forms (ZK_DMA and
     ZK_MF4, read when the handle is created): tables from constant memory (0),
     tables in LDS (1), rows by global-to-LDS loads into one (2) or two (3, the
     default) row buffers; with two buffers the products run as 4x4x4 blocks
-    (default) or as 16x16x4 tiles.  The irregular mask at nx = 150 gives three
+    (default) or as 16x16x4 tiles, and the 4x4x4 form either with the stage and
+    32-bit plane offsets fixed at compile time (default below 4 GiB) or in the
+    form used for larger arrays (ZK_FUSED_GENERIC=1).  The irregular mask at nx = 150 gives three
     62-column runs, the last partial, and 4-row segments put segment edges
     everywhere."""
     say(f"dg P={P} fused memory mode {mode}")
     monkeypatch.setenv("ZK_DMA", mode[0])
     monkeypatch.setenv("ZK_MF4", "0" if mode.endswith("16x16") else "1")
+    monkeypatch.setenv("ZK_FUSED_GENERIC", "1" if mode.endswith("generic") else "0")
     rng = np.random.default_rng(31 + P)
     ny, nx = 41, 150
     m = meshio.riemann_2d(64)
@@ -253,6 +256,7 @@ def test_dg_fused_memory_modes(P, mode, monkeypatch):
     s, o = dg_pair(m, P, zk.LIMITER_TVB, M=50.0)
     monkeypatch.delenv("ZK_DMA")
     monkeypatch.delenv("ZK_MF4")
+    monkeypatch.delenv("ZK_FUSED_GENERIC")
     dt = 0.5 * oracle.dg_cfl_limit(P) * m.da / o.max_speed()
     for variant in (104, 100):
         s.set_tuning(0, variant)

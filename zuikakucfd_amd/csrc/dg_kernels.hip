@@ -2519,7 +2519,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         // the kernel instance this stage launches
         // the default form with the stage and 32-bit plane offsets fixed at
         // compile time, when the state arrays are below 4 GiB
-        const bool fast = wg && H.dma == 3 && H.mf4 && (size_t)4 * (P + 1) * (P + 1) * H.g.np * 8 <= 0xFFFFFFFFull;
+        const bool fast = wg && H.dma == 3 && H.mf4 && !H.fused_generic &&
+                          (size_t)4 * (P + 1) * (P + 1) * H.g.np * 8 <= 0xFFFFFFFFull;
         A.row16 = (A.nx % 2 == 0) ? 1 : 0;
         const void *kern = (fast && stage == 0)       ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true>
                            : fast                     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true>

@@ -413,8 +413,10 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     // measurement overrides: ZK_NO_MFMA=1 (VALU cell kernel), ZK_DMA=0..3 (memory pipeline of
     // the fused DG kernel's workgroup form), ZK_MF4=0 (its 16x16x4 products instead of the
     // 4x4x4 blocks), ZK_TVB_LANE=0 (the 16-cell TVB scan instead of one lane per cell),
-    // ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is ignored)
+    // ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is ignored),
+    // ZK_FUSED_GENERIC=1 (the fused kernel form used for state arrays of 4 GiB and more)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
+    if (const char *v = std::getenv("ZK_FUSED_GENERIC")) H->fused_generic = v[0] == '1';
     if (const char *v = std::getenv("ZK_MF4")) H->mf4 = v[0] != '0';
     if (const char *v = std::getenv("ZK_TVB_LANE")) H->tvb_lane = v[0] != '0';
     if (const char *v = std::getenv("ZK_DMA"))
