@@ -205,8 +205,8 @@ def cpu_baseline(w: dict, m, dt: float, warmup: int = 5, steps: int = 20, max_s:
             "sample": f"{n} timed SSP-RK3 steps after {warmup} untimed of the same {m.ny}x{m.nx} problem and dt, "
                       f"the C oracle (OpenMP, {threads} threads) built {build}; {el:.1f} s",
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": aff,
-            "threads_note": "OMP_NUM_THREADS as set on this host (the GPU box gives one GPU's job 16 cores "
-                            "and sets it to 16); nproc counts the whole machine"}
+            "threads_note": "OMP_NUM_THREADS as this host sets it (a host that shares its cores among "
+                            "jobs sets it to the job's share); nproc as os.cpu_count() reports it"}
 
 
 def _cpu_baseline_child(args) -> int:
