@@ -226,7 +226,7 @@ def test_dg_random_mask(P, variant, nx):
 
 
 @pytest.mark.parametrize("P", [1, 2, 3])
-@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "3-16x16", "3-generic"])
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "3-62col", "3-16x16", "3-generic"])
 def test_dg_fused_memory_modes(P, mode, monkeypatch):
     """The fused kernel's memory pipelines and product forms (ZK_DMA and
     ZK_MF4, read when the handle is created): tables from constant memory (0),
@@ -234,13 +234,15 @@ def test_dg_fused_memory_modes(P, mode, monkeypatch):
     default) row buffers; with two buffers the products run as 4x4x4 blocks
     (default) or as 16x16x4 tiles, and the 4x4x4 form either with the stage and
     32-bit plane offsets fixed at compile time (default below 4 GiB) or in the
-    form used for larger arrays (ZK_FUSED_GENERIC=1).  The irregular mask at nx = 150 gives three
-    62-column runs, the last partial, and 4-row segments put segment edges
-    everywhere."""
+    form used for larger arrays (ZK_FUSED_GENERIC=1).  At P = 3 the default
+    runs on aligned 64-column blocks; "3-62col" (ZK_FUSED_AL=0) keeps the
+    62-column runs.  The irregular mask at nx = 150 gives three runs, the last
+    partial, and 4-row segments put segment edges everywhere."""
     say(f"dg P={P} fused memory mode {mode}")
     monkeypatch.setenv("ZK_DMA", mode[0])
     monkeypatch.setenv("ZK_MF4", "0" if mode.endswith("16x16") else "1")
     monkeypatch.setenv("ZK_FUSED_GENERIC", "1" if mode.endswith("generic") else "0")
+    monkeypatch.setenv("ZK_FUSED_AL", "0" if mode.endswith("62col") else "1")
     rng = np.random.default_rng(31 + P)
     ny, nx = 41, 150
     m = meshio.riemann_2d(64)
@@ -257,6 +259,7 @@ def test_dg_fused_memory_modes(P, mode, monkeypatch):
     monkeypatch.delenv("ZK_DMA")
     monkeypatch.delenv("ZK_MF4")
     monkeypatch.delenv("ZK_FUSED_GENERIC")
+    monkeypatch.delenv("ZK_FUSED_AL")
     dt = 0.5 * oracle.dg_cfl_limit(P) * m.da / o.max_speed()
     for variant in (104, 100):
         s.set_tuning(0, variant)
@@ -265,6 +268,42 @@ def test_dg_fused_memory_modes(P, mode, monkeypatch):
         Q = s.get_state()
         for c in range(4):
             assert rel_l1(Q[c], o.Q[c]) <= TOL, (variant, c)
+        np.testing.assert_array_equal(s.get_flags(), o.flags)
+
+
+@pytest.mark.parametrize("flux", [zk.FLUX_HLLC, zk.FLUX_HLL])
+def test_dg3_aligned_block_edges(flux):
+    """P = 3 on aligned 64-column blocks, where the faces at columns 63|64 and
+    127|128 are evaluated per row by one wave from the side columns' modes:
+    slip cells (their ghost-map entries negate the normal momentum) and
+    obstacle cells on both sides of those block edges, a periodic pair of
+    side walls, 5-row segments; state and TVB flags against the oracle over 12
+    steps, with HLLC and with HLL."""
+    say(f"dg P=3 aligned block edges flux {flux}")
+    rng = np.random.default_rng(7)
+    ny, nx = 46, 150
+    m = meshio.riemann_2d(64)
+    m.rho = 0.5 + rng.random((ny, nx)); m.u = rng.uniform(-1, 1, (ny, nx))
+    m.v = rng.uniform(-1, 1, (ny, nx)); m.p = 0.5 + rng.random((ny, nx))
+    b = meshio.ring_bctype(ny, nx)
+    b[3:ny - 3, :3] = -2          # periodic in x
+    b[3:ny - 3, nx - 3:] = -2
+    b[6:12, 60:64] = 1            # slip block ending at the first block edge
+    b[15:19, 64:67] = 1           # slip block starting at it
+    b[22:25, 126:128] = -1        # obstacle ending at the second block edge
+    b[30:33, 128:131] = -1        # and one starting at it
+    b[36:40, 62:66] = 1           # across the first edge
+    m.bctype = b
+    m.nx, m.ny = nx, ny
+    s, o = dg_pair(m, 3, zk.LIMITER_TVB, flux=flux, M=50.0)
+    s.set_tuning(0, 105)
+    dt = 0.5 * oracle.dg_cfl_limit(3) * m.da / o.max_speed()
+    for _ in range(3):
+        s.step(dt, 4)
+        o.step(dt, 4)
+        Q = s.get_state()
+        for c in range(4):
+            assert rel_l1(Q[c], o.Q[c]) <= TOL, c
         np.testing.assert_array_equal(s.get_flags(), o.flags)
 
 

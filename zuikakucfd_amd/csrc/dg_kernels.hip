@@ -1175,7 +1175,21 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // the stage combination carries no selects (0 reads A.stage).  O32: plane
 // offsets in 32 bits (state arrays below 4 GiB), which shortens the scalar
 // address arithmetic of the 3 x 4 (P+1)^2 plane accesses per row.
-template <int P, int MINW, bool WG, int MODE, bool F4, int SK = 0, bool O32 = false>
+//
+// AL (with WG, MODE 3, F4): aligned blocks.  Run r covers the 64 columns
+// [64 r, 64 r + 64) and updates all of them that lie in the update range, so
+// every 16-column segment a wave reads or writes is one 128-byte line and the
+// workgroup's plane rows start on 512-byte boundaries.  With the 62-column
+// stride of the default form each such segment straddles two lines and the
+// stores leave partial lines, which a probe of the same access pattern
+// without arithmetic measured at 475 against 348 us for the three arrays of
+// the second stage.  A block has 65 x faces for its 64 columns: the two at
+// its edges need the traces of the columns either side (64 r - 1 and
+// 64 r + 64), whose modes arrive with each row by one global-to-LDS load per
+// wave.  Wave 1 evaluates the two edge fluxes of row r after its
+// combination, where the fewest registers are live, and they join the edge
+// cells' S-face term when row r is finished in the next iteration.
+template <int P, int MINW, bool WG, int MODE, bool F4, int SK = 0, bool O32 = false, bool AL = false>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
     using Ofs = typename std::conditional<O32, uint32_t, size_t>::type;
@@ -1184,6 +1198,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     constexpr int NB = DB ? 2 : 1, NS = DB ? 4 : 2;  // row buffers, ghost-map row slots
     static_assert(WG || MODE == 0, "LDS tables need the workgroup form");
     static_assert(!F4 || MODE >= 1, "the 4x4-block form reads its tables from LDS");
+    static_assert(!AL || (WG && MODE == 3 && F4), "aligned blocks are a form of the default pipeline");
     constexpr size_t TAB_BYTES = F4 ? sizeof(MfmaTabs4) : sizeof(MfmaTabsF);
     __shared__ double tabs_lds[MODE >= 1 ? TAB_BYTES / 8 : 1];
     const MfmaTabsF &M = (MODE >= 1) ? *reinterpret_cast<const MfmaTabsF *>(tabs_lds) : c_mff[P - 1];
@@ -1192,7 +1207,15 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     __shared__ double lds_w[WG ? 4 : 1][4][4];   // [wave][g][c]: W trace of the wave's column 0
     __shared__ double lds_ge[WG ? 4 : 1][4][4];  // [wave][g][c]: projected flux of its column-15 face
     __shared__ double qbuf[DMA ? NB : 1][DMA ? 4 * NM : 1][DMA ? 64 : 1];  // rows: plane (c, mode) x 64 columns
-    __shared__ int32_t srcbuf[DMA ? NS : 1][DMA ? 64 : 1];               // ghost-map entries of row r: slot r % NS
+    // ghost-map entries of row r: slot r % NS; with AL two more, of the side columns
+    __shared__ int32_t srcbuf[DMA ? NS : 1][DMA ? (AL ? 66 : 64) : 1];
+    // AL: modes of the two side columns [variable][mode][side] per row buffer; the
+    // E trace of column 63 [g][c]; the side columns' traces [side][g][c]; and the
+    // projected edge fluxes of the pending row [side][py][c]
+    __shared__ double hbuf[AL ? NB : 1][AL ? 4 * NM * 2 : 1];
+    __shared__ double lds_e3[AL ? 4 : 1][4];
+    __shared__ double lds_ht[AL ? 2 : 1][4][4];
+    __shared__ double lds_bf[AL ? 2 : 1][4][4];
     if (A.S->done) return;
     const bool first = SK == 1 ? true : SK == 2 ? false : A.stage == 0;
     const int lane = threadIdx.x & 63;
@@ -1209,8 +1232,13 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         run = t % nrun;
         seg = t / nrun;
         const int x = 16 * wv + col;
-        j = 2 + run * 62 + x;
-        store_col = x >= 1 && x <= 62 && j <= A.nx - 4;
+        if (AL) {
+            j = 64 * run + x;
+            store_col = j >= 3 && j <= A.nx - 4;
+        } else {
+            j = 2 + run * 62 + x;
+            store_col = x >= 1 && x <= 62 && j <= A.nx - 4;
+        }
     } else {
         const int w = blockIdx.x * (blockDim.x >> 6) + wv;
         if (w >= nrun * nseg) return;  // whole wave
@@ -1266,10 +1294,13 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     // (l >> 1) and 32 + (l >> 1) of every plane of variable w; the source
     // cells come from the ghost map (slip signs are applied on reading).
     // Wave 0 also moves the ghost-map row two ahead into srcbuf.
-    const int jrun0 = 2 + run * 62;
+    const int jrun0 = AL ? 64 * run : 2 + run * 62;
     const int wvu = __builtin_amdgcn_readfirstlane(wv);
+    // ghost-map entry of block column x of a row, the column clamped into the
+    // grid (x = -1 and x = 64 are the AL side columns)
     auto col_src = [&](int row, int x) {
-        const int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
+        int jj = jrun0 + x < A.nx ? jrun0 + x : A.nx - 1;
+        jj = jj < 0 ? 0 : jj;
         return A.src[(size_t)row * A.nx + jj];
     };
     auto dma_row = [&](int row, int32_t e0, int32_t e1, Ofs npb) {
@@ -1311,9 +1342,33 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         __builtin_amdgcn_global_load_lds(
             (const void __attribute__((address_space(1))) *)(rowp + js),
             (void __attribute__((address_space(3))) *)&srcbuf[row % NS][0], 4, 0, 0);
+        if (AL && l < 2) {  // entries 64 and 65: the columns left and right of the block
+            int jh = l == 0 ? jrun0 - 1 : jrun0 + 64;
+            jh = jh < 0 ? 0 : (jh < A.nx ? jh : A.nx - 1);
+            __builtin_amdgcn_global_load_lds(
+                (const void __attribute__((address_space(1))) *)(rowp + jh),
+                (void __attribute__((address_space(3))) *)&srcbuf[row % NS][64], 4, 0, 0);
+        }
+    };
+    // AL: the modes of the two side columns of a row, one instruction per wave:
+    // lane l of wave w moves dword (l & 1) of side (l >> 1) & 1 of plane
+    // w NM + (l >> 2), from the side columns' ghost-map sources el and er
+    auto dma_halo = [&](int row, int32_t el, int32_t er, Ofs npb) {
+        int l;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+        if (4 * NM >= 64 || l < 4 * NM) {
+            const int32_t e = (l & 2) ? er : el;
+            const char *plane = reinterpret_cast<const char *>(plane_at(A.Qin, wvu * NM + (l >> 2), npb));
+            const uint32_t o = (uint32_t)(e >> kSrcShift) * 8u + (uint32_t)(l & 1) * 4u;
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o),
+                                             (void __attribute__((address_space(3))) *)&hbuf[DB ? (row & 1) : 0]
+                                                                                           [wvu * NM * 2],
+                                             4, 0, 0);
+        }
     };
     if (DMA) {
         dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), (Ofs)(np * 8));
+        if (AL) dma_halo(r0, col_src(r0, -1), col_src(r0, 64), (Ofs)(np * 8));
         if (wvu == 0) {
             dma_src(r0);
             if (r0 + 1 <= r1) dma_src(r0 + 1);
@@ -1388,6 +1443,22 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Qr[2][v] = -Qr[2][v];
         };
+        // the same for variable c alone (the aligned form at P = 3: fewer
+        // registers live where the volume terms and the combination read them)
+        auto reread_var = [&](int c, double q[4]) {
+            int l2 = lane;
+            asm volatile("" : "+v"(l2));
+            const int x = 16 * wvu + (l2 & 15);
+            const int32_t eo = srcbuf[r % NS][x];
+            auto qb = qbuf[DB ? (r & 1) : 0];
+            const int gq = (l2 >> 4) <= P ? (l2 >> 4) : 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) q[v] = (v <= P && slot) ? qb[c * NM + v * NQ + gq][x] : 0.0;
+            if ((c == 1 && (eo & 1)) || (c == 2 && (eo & 2)))
+#pragma unroll
+                for (int v = 0; v < 4; ++v) q[v] = -q[v];
+        };
+        constexpr bool PERVAR = AL && DB && P != 2;
         double Q[4][4];
         bool upd;  // this lane's cell of row r is updated
         if (DMA) {
@@ -1426,6 +1497,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     dma_row16(r + 1, npb);
                 else
                     dma_row(r + 1, sn[ln >> 1], sn[32 + (ln >> 1)], npb);
+                if (AL) dma_halo(r + 1, sn[64], sn[65], npb);
                 if (wvu == 0 && r + 2 <= r1) dma_src(r + 2);
             }
         } else {
@@ -1488,6 +1560,16 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // is above M h^2 / 4 (the y deviations then stay below 3/4 M h^2).  A
         // cell the TVB test can change is always marked.
         bool lin_big = false;
+        if (AL && r > r0 && (wvu == 0 || wvu == 3) && (wvu == 0 ? col == 0 : col == 15) && store_col && slot) {
+            // the edge face of row r-1's edge cell, projected by wave 1 after
+            // that row's combination: + L_v(-1) g on the left, - g on the right
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double gb = lds_bf[wvu == 0 ? 0 : 1][ln >> 4][c];
+#pragma unroll
+                for (int v = 0; v <= P; ++v) base[c][v] += fs[v] * ((wvu == 3 || (v & 1)) ? -gb : gb);
+            }
+        }
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
@@ -1575,6 +1657,9 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 if (col == 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_w[wvu][ln >> 4][c] = Tr[c][0];
+                if (AL && wvu == 3 && col == 15)  // for the block's right edge face
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) lds_e3[ln >> 4][c] = Tr[c][1];
                 lds_barrier();
                 if (col == 15 && wv < 3)
 #pragma unroll
@@ -1595,13 +1680,20 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 gW[c] = __shfl_up(gE[c], 1, 16);           // face (col-1 | col)
             }
             if (WG) {
-                if (col == 15)
+                if (col == 15) {
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_ge[wvu][ln >> 4][c] = gE[c];
+                    if (AL && wvu == 3)  // the block's right edge face joins at the finish
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) gE[c] = 0.0;
+                }
                 lds_barrier();
                 if (col == 0 && wv > 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wvu - 1][ln >> 4][c];
+                if (AL && col == 0 && wvu == 0)  // and its left edge face
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) gW[c] = 0.0;
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -1611,12 +1703,13 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         if constexpr (F4) {
             // the coefficients L_v(xi_a), w_a L'_v(xi_a), w_a L_v(xi_a) fold to
             // constants (instruction operands) in the unrolled loops
-            if (DB && P != 2) reread(Q);
+            if (DB && P != 2 && !PERVAR) reread(Q);
             // the state at the points (xi_a, eta_g): modes at eta_g (contracted
             // over py on the matrix core), then sum_v L_v(xi_a) over px
             double U[4][4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
+                if constexpr (PERVAR) reread_var(c, Q[c]);
                 double t[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) t[v] = v <= P ? mf4(M4.aL[ln], Q[c][v]) : 0.0;
@@ -1681,7 +1774,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         const double cdt = first ? dt : A.cb * dt;
 #pragma unroll
         for (int v = 0; v < 4; ++v) fs[v] = upd ? cdt * fdu[v] : 0.0;
-        if (DB && P != 2) reread(Q);
+        if (DB && P != 2 && !PERVAR) reread(Q);
         // ca qn + cb (Q + dt du) with du = fdu acc, as ca qn + cb Q + fs acc
         // (fs = cb dt fdu, 0 for cells not updated); the first stage is
         // Q + fs acc for updated cells and qn (not read otherwise) for the rest.
@@ -1701,16 +1794,57 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         } else if (store_col && slot) {
             if (first) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < 4; ++c) {
+                    if constexpr (PERVAR) reread_var(c, Q[c]);
 #pragma unroll
                     for (int v = 0; v <= P; ++v) base[c][v] = upd ? fs[v] * acc[c][v] + Q[c][v] : qn[c][v];
+                }
             } else {
                 const double ca = A.ca, cb = A.cb;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < 4; ++c) {
+                    if constexpr (PERVAR) reread_var(c, Q[c]);
 #pragma unroll
                     for (int v = 0; v <= P; ++v)
                         base[c][v] = fs[v] * acc[c][v] + (cb * Q[c][v] + ca * qn[c][v]);
+                }
+            }
+        }
+        if (AL && wvu == 1) {
+            // The two edge fluxes of row r.  Side-column traces first: lanes
+            // (g, col < 4) the E trace of the left side column for variable
+            // col, lanes (g, 4 <= col < 8) the W trace of the right one for
+            // variable col - 4, with the trace products of the rows (so the
+            // neighbouring block evaluates the same face from the same values).
+            int l2 = lane;
+            asm volatile("" : "+v"(l2));
+            const int g2 = l2 >> 4, c2 = l2 & 15, sd = (c2 >> 2) & 1, cv = c2 & 3;
+            const bool live = c2 < 8 && g2 <= P;
+            const int32_t eh = srcbuf[r % NS][64 + sd];
+            const bool neg = (cv == 1 && (eh & 1)) || (cv == 2 && (eh & 2));
+            double tr = 0.0;
+#pragma unroll
+            for (int v = 0; v <= P; ++v) {
+                double q = live ? hbuf[r & 1][(cv * NM + v * NQ + (g2 <= P ? g2 : 0)) * 2 + sd] : 0.0;
+                q = neg ? -q : q;
+                const double t = mf4(M4.aL[l2], q);
+                tr = v == 0 ? t : tr + ((sd == 1 && (v & 1)) ? -t : t);
+            }
+            if (c2 < 8) lds_ht[sd][g2][cv] = tr;
+            // lanes (g, 0): the left edge face at point g (side column | column 0);
+            // lanes (g, 1): the right one (column 63 | side column)
+            double Lx[4], Rx[4], Fx[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                Lx[c] = c2 == 0 ? lds_ht[0][g2][c] : lds_e3[g2][c];
+                Rx[c] = c2 == 0 ? lds_w[0][g2][c] : lds_ht[1][g2][c];
+            }
+            if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
+            else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double gb = mf4(M4.aWL[l2], Fx[c]);  // onto py = g
+                if (c2 < 2) lds_bf[c2][g2][c] = gb;
             }
         }
         kp = k;
@@ -2515,14 +2649,20 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const bool wg = H.tune == 0 || H.tune >= 100;
         int R = H.tune >= 100 ? H.tune - 100 : (H.tune >= 10 ? H.tune - 10 : 0);
         const int width = wg ? 62 : 14;                 // updated columns per wave (per workgroup for wg)
-        const int nrun = (A.nxu + width - 1) / width;
         // the kernel instance this stage launches
         // the default form with the stage and 32-bit plane offsets fixed at
-        // compile time, when the state arrays are below 4 GiB
+        // compile time, when the state arrays are below 4 GiB; at P = 3 on
+        // aligned 64-column blocks unless ZK_FUSED_AL=0 (at P = 2 the extra
+        // edge-flux call per row costs more than the aligned accesses save:
+        // 2048^2 1.13 against 1.02 ms per launch)
         const bool fast = wg && H.dma == 3 && H.mf4 && !H.fused_generic &&
                           (size_t)4 * (P + 1) * (P + 1) * H.g.np * 8 <= 0xFFFFFFFFull;
+        const bool al = fast && H.fused_al && P == 3;
+        const int nrun = al ? (A.nx - 4) / 64 + 1 : (A.nxu + width - 1) / width;
         A.row16 = (A.nx % 2 == 0) ? 1 : 0;
-        const void *kern = (fast && stage == 0)       ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true>
+        const void *kern = (al && stage == 0)         ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true, true>
+                           : al                       ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true, true>
+                           : (fast && stage == 0)     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true>
                            : fast                     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true>
                            : (wg && H.dma == 3 && H.mf4) ? (const void *)dg_fused_kernel<P, 2, true, 3, true>
                            : (wg && H.dma == 3)         ? (const void *)dg_fused_kernel<P, 2, true, 3, false>
@@ -2534,8 +2674,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             // Whole rounds of the device: as many workgroups (or single waves)
             // as the CUs hold at the kernel's occupancy.  The fewest rounds with
             // R <= 64 keep halo rows few and leave no part-filled last round.
-            static std::atomic<int> occ_cache[8];  // per kernel instance of this P (0 = not known yet)
-            const int slot_i = fast ? (stage == 0 ? 6 : 7)
+            static std::atomic<int> occ_cache[10];  // per kernel instance of this P (0 = not known yet)
+            const int slot_i = al ? (stage == 0 ? 8 : 9) : fast ? (stage == 0 ? 6 : 7)
                                : !wg ? 5 : H.dma == 3 ? (H.mf4 ? 0 : 1) : H.dma == 2 ? 2 : H.dma == 1 ? 3 : 4;
             int per_cu = occ_cache[slot_i].load();
             if (per_cu < 1) {

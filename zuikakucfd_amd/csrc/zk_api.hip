@@ -419,6 +419,7 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     if (const char *v = std::getenv("ZK_FUSED_GENERIC")) H->fused_generic = v[0] == '1';
     if (const char *v = std::getenv("ZK_MF4")) H->mf4 = v[0] != '0';
     if (const char *v = std::getenv("ZK_TVB_LANE")) H->tvb_lane = v[0] != '0';
+    if (const char *v = std::getenv("ZK_FUSED_AL")) H->fused_al = v[0] != '0';
     if (const char *v = std::getenv("ZK_DMA"))
         if (v[0] >= '0' && v[0] <= '3' && v[1] == 0) H->dma = v[0] - '0';
     if (const char *v = std::getenv("ZK_TUNE")) {

@@ -58,6 +58,7 @@ struct Handle {
     bool mf4 = true;          // fused DG kernel (two row buffers) on 4x4x4 matrix-core blocks (ZK_MF4=0: 16x16x4)
     bool fused_generic = false;  // ZK_FUSED_GENERIC=1: the fused kernel's form for state arrays of 4 GiB and more
                                  // (stage read at run time, 64-bit plane offsets) at every size
+    bool fused_al = true;     // fused DG kernel on aligned 64-column blocks (ZK_FUSED_AL=0: the 62-column runs)
     bool tvb_lane = true;     // TVB after the fused kernel: one lane per cell (ZK_TVB_LANE=0: the 16-cell scan)
     bool tvb_linear = false;  // ZK_LIMITER_TVB_LINEAR: the TVB test on the linear modes alone at every P
     int fv_tile = 0;          // WENO tile shape: 0 auto, 1 = 16x15, 2 = 32x15
