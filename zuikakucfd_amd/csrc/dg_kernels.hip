@@ -1186,9 +1186,11 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // the second stage.  A block has 65 x faces for its 64 columns: the two at
 // its edges need the traces of the columns either side (64 r - 1 and
 // 64 r + 64), whose modes arrive with each row by one global-to-LDS load per
-// wave.  Wave 1 evaluates the two edge fluxes of row r after its
-// combination, where the fewest registers are live, and they join the edge
-// cells' S-face term when row r is finished in the next iteration.
+// wave.  At the top of each row wave 1 evaluates the two edge fluxes from
+// LDS alone (side columns and columns 0 and 63, with the same trace products
+// as the rows, so both blocks at an edge compute the same flux bit for bit),
+// and the edge cells take them into the same sums as their inner faces
+// (a uniform state stays exactly uniform).
 template <int P, int MINW, bool WG, int MODE, bool F4, int SK = 0, bool O32 = false, bool AL = false>
 __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, int nrun)
 {
@@ -1209,12 +1211,12 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     __shared__ double qbuf[DMA ? NB : 1][DMA ? 4 * NM : 1][DMA ? 64 : 1];  // rows: plane (c, mode) x 64 columns
     // ghost-map entries of row r: slot r % NS; with AL two more, of the side columns
     __shared__ int32_t srcbuf[DMA ? NS : 1][DMA ? (AL ? 66 : 64) : 1];
-    // AL: modes of the two side columns [variable][mode][side] per row buffer; the
-    // E trace of column 63 [g][c]; the side columns' traces [side][g][c]; and the
-    // projected edge fluxes of the pending row [side][py][c]
+    // AL: modes of the two side columns [variable][mode][side] per row buffer;
+    // the edge traces of the row [which][g][c] (E of the left side column, W of
+    // the right one, W of column 0, E of column 63); and the row's projected
+    // edge fluxes [side][py][c]
     __shared__ double hbuf[AL ? NB : 1][AL ? 4 * NM * 2 : 1];
-    __shared__ double lds_e3[AL ? 4 : 1][4];
-    __shared__ double lds_ht[AL ? 2 : 1][4][4];
+    __shared__ double lds_ht[AL ? 4 : 1][4][4];
     __shared__ double lds_bf[AL ? 2 : 1][4][4];
     if (A.S->done) return;
     const bool first = SK == 1 ? true : SK == 2 ? false : A.stage == 0;
@@ -1504,6 +1506,46 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             load(r, Q);
             upd = A.upd[k] != 0;
         }
+        if (AL && inner && wvu == 1) {
+            // The two edge fluxes of row r, from LDS alone.  Traces: lanes
+            // (g, col) of block b = col / 4 hold variable col % 4 of the E trace
+            // of the left side column (b = 0), the W trace of the right one
+            // (b = 1), the W trace of column 0 (b = 2) and the E trace of
+            // column 63 (b = 3), each at point g, with the rows' trace products.
+            int l2 = lane;
+            asm volatile("" : "+v"(l2));
+            const int g2 = l2 >> 4, bk = (l2 >> 2) & 3, cv = l2 & 3, gq = g2 <= P ? g2 : 0;
+            const int32_t eh = bk < 2 ? srcbuf[r % NS][64 + bk] : srcbuf[r % NS][bk == 2 ? 0 : 63];
+            const bool neg = (cv == 1 && (eh & 1)) || (cv == 2 && (eh & 2));
+            auto qb = qbuf[r & 1];
+            double tr = 0.0;
+#pragma unroll
+            for (int v = 0; v <= P; ++v) {
+                const int pl = cv * NM + v * NQ + gq;
+                double q = g2 > P ? 0.0 : bk < 2 ? hbuf[r & 1][pl * 2 + bk] : qb[pl][bk == 2 ? 0 : 63];
+                q = neg ? -q : q;
+                const double t = mf4(M4.aL[l2], q);
+                // W traces (blocks 1, 2) alternate the sign of the odd modes
+                tr = v == 0 ? t : tr + (((bk == 1 || bk == 2) && (v & 1)) ? -t : t);
+            }
+            lds_ht[bk][g2][cv] = tr;
+            // lanes (g, 0): the left edge face at point g (side column | column 0);
+            // lanes (g, 1): the right one (column 63 | side column)
+            const int c2 = l2 & 15;
+            double Lx[4], Rx[4], Fx[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                Lx[c] = lds_ht[c2 == 0 ? 0 : 3][g2][c];
+                Rx[c] = lds_ht[c2 == 0 ? 2 : 1][g2][c];
+            }
+            if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
+            else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double gb = mf4(M4.aWL[l2], Fx[c]);  // onto py = g
+                if (c2 < 2) lds_bf[c2][g2][c] = gb;
+            }
+        }
         // face traces of this row: element side (W, E, N, S) at quadrature point g
         d4 Tr[4];
 #pragma unroll
@@ -1560,16 +1602,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         // is above M h^2 / 4 (the y deviations then stay below 3/4 M h^2).  A
         // cell the TVB test can change is always marked.
         bool lin_big = false;
-        if (AL && r > r0 && (wvu == 0 || wvu == 3) && (wvu == 0 ? col == 0 : col == 15) && store_col && slot) {
-            // the edge face of row r-1's edge cell, projected by wave 1 after
-            // that row's combination: + L_v(-1) g on the left, - g on the right
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double gb = lds_bf[wvu == 0 ? 0 : 1][ln >> 4][c];
-#pragma unroll
-                for (int v = 0; v <= P; ++v) base[c][v] += fs[v] * ((wvu == 3 || (v & 1)) ? -gb : gb);
-            }
-        }
         if (r > r0 && store_col && slot) {
             // finish row r-1: its S face is this row's N face
 #pragma unroll
@@ -1657,9 +1689,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 if (col == 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_w[wvu][ln >> 4][c] = Tr[c][0];
-                if (AL && wvu == 3 && col == 15)  // for the block's right edge face
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) lds_e3[ln >> 4][c] = Tr[c][1];
                 lds_barrier();
                 if (col == 15 && wv < 3)
 #pragma unroll
@@ -1680,20 +1709,22 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                 gW[c] = __shfl_up(gE[c], 1, 16);           // face (col-1 | col)
             }
             if (WG) {
-                if (col == 15) {
+                if (col == 15)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) lds_ge[wvu][ln >> 4][c] = gE[c];
-                    if (AL && wvu == 3)  // the block's right edge face joins at the finish
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) gE[c] = 0.0;
-                }
                 lds_barrier();
                 if (col == 0 && wv > 0)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) gW[c] = lds_ge[wvu - 1][ln >> 4][c];
-                if (AL && col == 0 && wvu == 0)  // and its left edge face
+                if (AL && (wvu == 0 ? col == 0 : (wvu == 3 && col == 15))) {
+                    // the block's edge faces, from wave 1 (written before the first x-face barrier)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) gW[c] = 0.0;
+                    for (int c = 0; c < 4; ++c) {
+                        const double gb = lds_bf[wvu == 0 ? 0 : 1][ln >> 4][c];
+                        if (wvu == 0) gW[c] = gb;
+                        else gE[c] = gb;
+                    }
+                }
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
@@ -1808,43 +1839,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
                     for (int v = 0; v <= P; ++v)
                         base[c][v] = fs[v] * acc[c][v] + (cb * Q[c][v] + ca * qn[c][v]);
                 }
-            }
-        }
-        if (AL && wvu == 1) {
-            // The two edge fluxes of row r.  Side-column traces first: lanes
-            // (g, col < 4) the E trace of the left side column for variable
-            // col, lanes (g, 4 <= col < 8) the W trace of the right one for
-            // variable col - 4, with the trace products of the rows (so the
-            // neighbouring block evaluates the same face from the same values).
-            int l2 = lane;
-            asm volatile("" : "+v"(l2));
-            const int g2 = l2 >> 4, c2 = l2 & 15, sd = (c2 >> 2) & 1, cv = c2 & 3;
-            const bool live = c2 < 8 && g2 <= P;
-            const int32_t eh = srcbuf[r % NS][64 + sd];
-            const bool neg = (cv == 1 && (eh & 1)) || (cv == 2 && (eh & 2));
-            double tr = 0.0;
-#pragma unroll
-            for (int v = 0; v <= P; ++v) {
-                double q = live ? hbuf[r & 1][(cv * NM + v * NQ + (g2 <= P ? g2 : 0)) * 2 + sd] : 0.0;
-                q = neg ? -q : q;
-                const double t = mf4(M4.aL[l2], q);
-                tr = v == 0 ? t : tr + ((sd == 1 && (v & 1)) ? -t : t);
-            }
-            if (c2 < 8) lds_ht[sd][g2][cv] = tr;
-            // lanes (g, 0): the left edge face at point g (side column | column 0);
-            // lanes (g, 1): the right one (column 63 | side column)
-            double Lx[4], Rx[4], Fx[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                Lx[c] = c2 == 0 ? lds_ht[0][g2][c] : lds_e3[g2][c];
-                Rx[c] = c2 == 0 ? lds_w[0][g2][c] : lds_ht[1][g2][c];
-            }
-            if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
-            else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double gb = mf4(M4.aWL[l2], Fx[c]);  // onto py = g
-                if (c2 < 2) lds_bf[c2][g2][c] = gb;
             }
         }
         kp = k;
