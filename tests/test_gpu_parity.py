@@ -235,8 +235,8 @@ def test_dg_fused_memory_modes(P, mode, monkeypatch):
     (default) or as 16x16x4 tiles, and the 4x4x4 form either with the stage and
     32-bit plane offsets fixed at compile time (default below 4 GiB) or in the
     form used for larger arrays (ZK_FUSED_GENERIC=1).  At P = 3 the default
-    runs on aligned 64-column blocks; "3-62col" (ZK_FUSED_AL=0) keeps the
-    62-column runs.  The irregular mask at nx = 150 gives three runs, the last
+    runs its two later stages on aligned 64-column blocks; "3-62col"
+    (ZK_FUSED_AL=0) keeps the 62-column runs in every stage.  The irregular mask at nx = 150 gives three runs, the last
     partial, and 4-row segments put segment edges everywhere."""
     say(f"dg P={P} fused memory mode {mode}")
     monkeypatch.setenv("ZK_DMA", mode[0])
@@ -305,6 +305,42 @@ def test_dg3_aligned_block_edges(flux):
         for c in range(4):
             assert rel_l1(Q[c], o.Q[c]) <= TOL, c
         np.testing.assert_array_equal(s.get_flags(), o.flags)
+
+
+def test_dg3_aligned_blocks_bitwise_equal_62col(monkeypatch):
+    """The aligned-block stages evaluate every face flux from the same traces
+    with the same flux routine and add it in the same order as the 62-column
+    runs, so the two forms give the same state and flags bit for bit: 40 steps
+    at 200 x 150 with slip walls, obstacles and periodic sides, graph replay
+    on."""
+    say("dg P=3 aligned blocks == 62-column runs, bitwise")
+    rng = np.random.default_rng(11)
+    ny, nx = 200, 150
+    m = meshio.riemann_2d(64)
+    m.rho = 0.5 + rng.random((ny, nx)); m.u = rng.uniform(-1, 1, (ny, nx))
+    m.v = rng.uniform(-1, 1, (ny, nx)); m.p = 0.5 + rng.random((ny, nx))
+    b = meshio.ring_bctype(ny, nx)
+    b[3:ny - 3, :3] = -2
+    b[3:ny - 3, nx - 3:] = -2
+    b[40:50, 60:70] = 1
+    b[100:104, 125:131] = -1
+    m.bctype = b
+    m.nx, m.ny = nx, ny
+    runs = []
+    for al in ("1", "0"):
+        monkeypatch.setenv("ZK_FUSED_AL", al)
+        s = zk.Solver(method=zk.METHOD_DG, P=3, limiter=zk.LIMITER_TVB, tvb_M=50.0)
+        s.load(m.da, m.gamma, m.rho, m.u, m.v, m.p, m.bctype)
+        runs.append(s)
+    monkeypatch.delenv("ZK_FUSED_AL")
+    dt = 0.5 * oracle.dg_cfl_limit(3) * m.da / runs[0].max_speed()
+    for _ in range(4):
+        for s in runs:
+            s.step(dt, 10)
+        np.testing.assert_array_equal(runs[0].get_state(), runs[1].get_state())
+        np.testing.assert_array_equal(runs[0].get_flags(), runs[1].get_flags())
+    for s in runs:
+        s.close()
 
 
 @pytest.mark.parametrize("P", [1, 2, 3])

@@ -2645,17 +2645,19 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         const int width = wg ? 62 : 14;                 // updated columns per wave (per workgroup for wg)
         // the kernel instance this stage launches
         // the default form with the stage and 32-bit plane offsets fixed at
-        // compile time, when the state arrays are below 4 GiB; at P = 3 on
-        // aligned 64-column blocks unless ZK_FUSED_AL=0 (at P = 2 the extra
-        // edge-flux call per row costs more than the aligned accesses save:
-        // 2048^2 1.13 against 1.02 ms per launch)
+        // compile time, when the state arrays are below 4 GiB; at P = 3 the
+        // two later stages on aligned 64-column blocks unless ZK_FUSED_AL=0
+        // (the same results bit for bit).  They read U^n as well, and the
+        // aligned accesses save more there than the extra edge-flux call per
+        // row costs; the first stage and P = 2 are faster without (1024^2 P=3
+        // mid-run: stage 1 412 against 363 us, stages 2-3 439 against 480;
+        // 2048^2 P=2 1.13 against 1.02 ms)
         const bool fast = wg && H.dma == 3 && H.mf4 && !H.fused_generic &&
                           (size_t)4 * (P + 1) * (P + 1) * H.g.np * 8 <= 0xFFFFFFFFull;
-        const bool al = fast && H.fused_al && P == 3;
+        const bool al = fast && H.fused_al && P == 3 && stage > 0;
         const int nrun = al ? (A.nx - 4) / 64 + 1 : (A.nxu + width - 1) / width;
         A.row16 = (A.nx % 2 == 0) ? 1 : 0;
-        const void *kern = (al && stage == 0)         ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true, true>
-                           : al                       ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true, true>
+        const void *kern = al                         ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true, true>
                            : (fast && stage == 0)     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 1, true>
                            : fast                     ? (const void *)dg_fused_kernel<P, 2, true, 3, true, 2, true>
                            : (wg && H.dma == 3 && H.mf4) ? (const void *)dg_fused_kernel<P, 2, true, 3, true>
@@ -2668,8 +2670,8 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             // Whole rounds of the device: as many workgroups (or single waves)
             // as the CUs hold at the kernel's occupancy.  The fewest rounds with
             // R <= 64 keep halo rows few and leave no part-filled last round.
-            static std::atomic<int> occ_cache[10];  // per kernel instance of this P (0 = not known yet)
-            const int slot_i = al ? (stage == 0 ? 8 : 9) : fast ? (stage == 0 ? 6 : 7)
+            static std::atomic<int> occ_cache[9];  // per kernel instance of this P (0 = not known yet)
+            const int slot_i = al ? 8 : fast ? (stage == 0 ? 6 : 7)
                                : !wg ? 5 : H.dma == 3 ? (H.mf4 ? 0 : 1) : H.dma == 2 ? 2 : H.dma == 1 ? 3 : 4;
             int per_cu = occ_cache[slot_i].load();
             if (per_cu < 1) {
