@@ -593,8 +593,10 @@ def main() -> int:
     if args.workload.startswith("dg3_tvb"):
         # what the counters of this kernel show (committed rocprofv3 summaries)
         line["roofline"]["measured_limit"] = (
-            "SIMD issue, not HBM: the fused kernel's waves issue 26-34% of their cycles and wait to issue "
-            "45-52% (profiles/dg3_tvb_riemann1024_stall_pmc.txt); one wave per SIMD runs 0.89x as fast as two")
+            "memory access pattern and FP64 issue together, not HBM bandwidth: with every global access kept in "
+            "the L2 the same instruction stream runs at about 306 us per launch against about 420 measured; the "
+            "waves issue 28-34% of their cycles, wait to issue 36-45% and wait on memory or barriers 21-36% "
+            "(profiles/dg3_tvb_riemann1024_stall_pmc.txt)")
     if args.resume_state:
         line["resumed"] = ("the timed window started from the state a --save-state run reached at step "
                            f"{args.warmup} (the untimed advance skipped)")
