@@ -225,15 +225,15 @@ __device__ __forceinline__ void euler_fg(const double U[4], double gam, double F
 
 template <int P>
 __device__ __forceinline__ void load_cell(const double *__restrict__ Q, const int32_t *__restrict__ src,
-                                          size_t np, size_t k, double A[4][(P + 1) * (P + 1)])
+                                          const StLay &L, size_t k, double A[4][(P + 1) * (P + 1)])
 {
     constexpr int NM = (P + 1) * (P + 1);
     const int32_t e = src[k];
-    const size_t s = (size_t)(e >> kSrcShift);
+    const size_t s = L.cell((size_t)(e >> kSrcShift));
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) A[c][m] = Q[(size_t)(c * NM + m) * np + s];
+        for (int m = 0; m < NM; ++m) A[c][m] = Q[(size_t)(c * NM + m) * L.ps + s];
     if (e & 1)
 #pragma unroll
         for (int m = 0; m < NM; ++m) A[1][m] = -A[1][m];
@@ -265,10 +265,11 @@ struct DgArgs {
     int stage, flux, limiter;
     int zs_ext;       // positivity option: the scaling also at the face points and corners
     int row16;        // rows may be staged with 16-byte loads (nx even: every row segment 16-byte aligned)
+    StLay L;          // layout of Qn, Qin and Qout (np above counts cells: per-cell arrays such as the means)
 };
 
-// Plane `pl` of a plane-major array, `stride` bytes apart.  With a 32-bit
-// stride (arrays below 4 GiB) the offset is one 32-bit scalar multiply.
+// Plane `pl` of a state array whose planes lie `stride` bytes apart.  With a
+// 32-bit stride (arrays below 4 GiB) the offset is one 32-bit scalar multiply.
 template <typename Ofs, typename T>
 __device__ __forceinline__ T *plane_at(T *base, int pl, Ofs stride)
 {
@@ -277,12 +278,12 @@ __device__ __forceinline__ T *plane_at(T *base, int pl, Ofs stride)
 }
 
 template <int P, bool X>
-__device__ __forceinline__ void load_trace(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
+__device__ __forceinline__ void load_trace(const double *__restrict__ Q, const int32_t *__restrict__ src, const StLay &L,
                                            size_t k, bool plus, double C[4][P + 1])
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const int32_t e = src[k];
-    const size_t s = (size_t)(e >> kSrcShift);
+    const size_t s = L.cell((size_t)(e >> kSrcShift));
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
 #pragma unroll
@@ -291,7 +292,7 @@ __device__ __forceinline__ void load_trace(const double *__restrict__ Q, const i
         for (int px = 0; px < NQ; ++px)
 #pragma unroll
             for (int py = 0; py < NQ; ++py) {
-                const double a = Q[(size_t)(c * NM + px * NQ + py) * np + s];
+                const double a = Q[(size_t)(c * NM + px * NQ + py) * L.ps + s];
                 const int t = X ? py : px;          // direction along the face
                 const int o = X ? px : py;          // direction across the face
                 if (plus || (o & 1) == 0) C[c][t] += a;
@@ -331,8 +332,8 @@ __global__ __launch_bounds__(128) void dg_face_kernel(DgArgs A)
     // at xi = +1 is sum_py (sum_px a[px][py]) L_py(eta) and the right cell's
     // at xi = -1 weights a[px][py] by (-1)^px; on a y face px and py swap.
     double CL[4][NQ], CR[4][NQ];
-    load_trace<P, X>(A.Qin, A.src, A.np, kl, true, CL);
-    load_trace<P, X>(A.Qin, A.src, A.np, kr, false, CR);
+    load_trace<P, X>(A.Qin, A.src, A.L, kl, true, CL);
+    load_trace<P, X>(A.Qin, A.src, A.L, kr, false, CR);
 
     double g[4][NQ];
 #pragma unroll
@@ -516,10 +517,11 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
     const int i = ii + 3, j = jj + 3;
     const size_t k = (size_t)i * A.nx + j;
     const size_t np = A.np;
+    const size_t sk = A.L.cell(k);  // the cell in the state arrays
     const bool upd = A.upd[k] != 0;
 
     double Q[4][NM];   // filled input of the cell, then its output
-    load_cell<P>(A.Qin, A.src, np, k, Q);
+    load_cell<P>(A.Qin, A.src, A.L, k, Q);
     double rhs[4][NM];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             const double dU = rhs[c][m] * (T.invm[m] * inv2h);
-            const double qn = A.Qn[(size_t)(c * NM + m) * np + k];
+            const double qn = A.Qn[(size_t)(c * NM + m) * A.L.ps + sk];
             Q[c][m] = (A.stage == 0) ? qn + dt * dU : A.ca * qn + A.cb * (Q[c][m] + dt * dU);
         }
     if (A.limiter == ZK_LIMITER_PERSSON_ZS) {
@@ -615,7 +617,7 @@ __global__ __launch_bounds__(128) void dg_cell_kernel(DgArgs A)
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) A.Qout[(size_t)(c * NM + m) * np + k] = Q[c][m];
+        for (int m = 0; m < NM; ++m) A.Qout[(size_t)(c * NM + m) * A.L.ps + sk] = Q[c][m];
 }
 
 __device__ __forceinline__ double minmod3(double a, double b, double c)
@@ -743,7 +745,7 @@ template <int P>
 __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, const double *__restrict__ M,
                                                      const int32_t *__restrict__ src,
                                                      const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
-                                                     int ny, int nx, size_t np, double Mh2, double gamma,
+                                                     int ny, int nx, size_t np, StLay L, double Mh2, double gamma,
                                                      const StepScalars *__restrict__ S, bool lin, bool ext)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
@@ -751,11 +753,12 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nxu * nyu || S->done) return;
     const size_t k = interior_cell(t, nx, nxu);
+    const size_t sk = L.cell(k);
     double C[4][NM];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * L.ps + sk];
     int flag = 0;
     if (bct[k] == 0) {
         constexpr int mxm = NQ, mym = 1;
@@ -777,7 +780,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * L.ps + sk] = C[c][m];
     }
 }
 
@@ -785,7 +788,7 @@ __global__ __launch_bounds__(128) void dg_tvb_kernel(double *__restrict__ Q, con
 // (computeMaxSpeedDG_omp, src_DG/solver.cpp:26-64).
 template <int P>
 __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restrict__ Q,
-                                                           const int32_t *__restrict__ bct, size_t np,
+                                                           const int32_t *__restrict__ bct, size_t np, StLay L,
                                                            double gam, unsigned long long *__restrict__ out)
 {
     constexpr int NM = (P + 1) * (P + 1);
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(256) void dg_max_speed_kernel(const double *__restr
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int m = 0; m < NM; ++m) A[c][m] = Q[(size_t)(c * NM + m) * np + k];
+            for (int m = 0; m < NM; ++m) A[c][m] = Q[L.at(c * NM + m, k)];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             double U[4] = {0.0, 0.0, 0.0, 0.0};
@@ -825,7 +828,8 @@ struct RingArgs {
     double *Qout;
     const int32_t *src, *ring;
     int nring, nm;
-    size_t np;
+    size_t np;  // cells per plane of M
+    StLay L;    // layout of Qn, Qin, Qout
     double ca, cb;
     int stage;
     double *M;  // means of the limiter (mode 0 of every variable), or null
@@ -840,11 +844,11 @@ __device__ __forceinline__ void ring_item(const RingArgs &R, size_t t)
     const int32_t e = R.src[k];
     const size_t s = (size_t)(e >> kSrcShift);
     const int var = (int)p / R.nm;
-    const double qn = R.Qn[p * R.np + k];
-    double qf = R.Qin[p * R.np + s];
+    const double qn = R.Qn[R.L.at((int)p, k)];
+    double qf = R.Qin[R.L.at((int)p, s)];
     if ((var == 1 && (e & 1)) || (var == 2 && (e & 2))) qf = -qf;
     const double out = (R.stage == 0) ? qn + 0.0 : R.ca * qn + R.cb * (qf + 0.0);
-    R.Qout[p * R.np + k] = out;
+    R.Qout[R.L.at((int)p, k)] = out;
     if (R.M && (int)p % R.nm == 0) R.M[(size_t)var * R.np + k] = out;
 }
 
@@ -925,12 +929,12 @@ __global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
 
     // this lane's modes (px = v, py = g) of the filled input cell
     const int32_t e = A.src[k];
-    const size_t sk = (size_t)(e >> kSrcShift);
+    const size_t sk = A.L.cell((size_t)(e >> kSrcShift));
     double Qr[4][4];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Qr[c][v] = A.Qin[(size_t)(c * NM + 4 * v + g) * np + sk];
+        for (int v = 0; v < 4; ++v) Qr[c][v] = A.Qin[(size_t)(c * NM + 4 * v + g) * A.L.ps + sk];
     if (e & 1)
 #pragma unroll
         for (int v = 0; v < 4; ++v) Qr[1][v] = -Qr[1][v];
@@ -980,7 +984,7 @@ __global__ __launch_bounds__(256, MINW) void dg_cell_mfma3_kernel(DgArgs A)
             if (!upd) rhs = 0.0;
             const int m = 4 * v + g;
             const double dU = rhs * (T.invm[m] * inv2h);
-            const size_t idx = (size_t)(c * NM + m) * np + k;
+            const size_t idx = (size_t)(c * NM + m) * A.L.ps + A.L.rc(ii + 3, jj + 3);
             const double qn = A.Qn[idx];
             const double out = (A.stage == 0) ? qn + dt * dU : A.ca * qn + A.cb * (Qr[c][v] + dt * dU);
             if (live) {
@@ -1261,6 +1265,9 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     const int r0 = 3 + seg * R;
     const int r1 = (r0 + R < A.ny - 3) ? r0 + R : A.ny - 3;  // rows [r0, r1), halo rows r0-1 and r1
     const size_t np = A.np;
+    const StLay L = A.L;
+    // element offset of plane 0 of cell kk, 32-bit when the state arrays are below 4 GiB
+    auto cel = [&](size_t kk) { return (Ofs)L.cell(kk); };
     const bool slot = g <= P;                         // this lane group holds real modes
     const double gam = A.gamma, dt = A.S->dt, inv2h = 0.5 / A.da;
     // per-slot factor of dU in the combination: dt (stage 1) or cb*dt, times invMass * 0.5/h
@@ -1273,17 +1280,18 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     }
 
     // Addressing: plane (c, px = v) is a wave-uniform pointer; a lane adds the
-    // 32-bit element offset g*np + cell, so addresses need no 64-bit VGPR pairs.
-    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    // element offset g*ps + cell, 32-bit below 4 GiB so that addresses need no
+    // 64-bit VGPR pairs.
+    const Ofs gofs = (Ofs)((size_t)(slot ? g : 0) * L.ps);
     auto load = [&](int row, double Q[4][4]) {
         const size_t k = (size_t)row * A.nx + jc;
         const int32_t e = A.src[k];
-        const uint32_t o = gofs + (uint32_t)(e >> kSrcShift);
+        const Ofs o = gofs + cel((size_t)(e >> kSrcShift));
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                Q[c][v] = (v <= P && slot) ? (A.Qin + (size_t)(c * NM + v * NQ) * np)[o] : 0.0;
+                Q[c][v] = (v <= P && slot) ? (A.Qin + (size_t)(c * NM + v * NQ) * L.ps)[o] : 0.0;
         if (e & 1)
 #pragma unroll
             for (int v = 0; v < 4; ++v) Q[1][v] = -Q[1][v];
@@ -1308,7 +1316,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     auto dma_row = [&](int row, int32_t e0, int32_t e1, Ofs npb) {
         auto qb = qbuf[DB ? (row & 1) : 0];
         const uint32_t b = (uint32_t)(lane & 1) * 4u;
-        const uint32_t o0 = (uint32_t)(e0 >> kSrcShift) * 8u + b, o1 = (uint32_t)(e1 >> kSrcShift) * 8u + b;
+        const Ofs o0 = cel((size_t)(e0 >> kSrcShift)) * 8u + b, o1 = cel((size_t)(e1 >> kSrcShift)) * 8u + b;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             const int pl = wvu * NM + m;
@@ -1325,7 +1333,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
     // Every such segment ends inside its row (column jrun0 + 63 < nx).
     auto dma_row16 = [&](int row, Ofs npb) {
         auto qb = qbuf[DB ? (row & 1) : 0];
-        const Ofs o = (Ofs)(((size_t)row * A.nx + jrun0) * 8 + (size_t)(lane & 31) * 16) + ((lane >> 5) ? npb : (Ofs)0);
+        const Ofs o = (Ofs)(L.rc(row, jrun0) * 8 + (size_t)(lane & 31) * 16) + ((lane >> 5) ? npb : (Ofs)0);
 #pragma unroll
         for (int m = 0; m < NM; m += 2) {
             const char *plane = reinterpret_cast<const char *>(plane_at(A.Qin, wvu * NM + m, npb));
@@ -1361,7 +1369,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         if (4 * NM >= 64 || l < 4 * NM) {
             const int32_t e = (l & 2) ? er : el;
             const char *plane = reinterpret_cast<const char *>(plane_at(A.Qin, wvu * NM + (l >> 2), npb));
-            const uint32_t o = (uint32_t)(e >> kSrcShift) * 8u + (uint32_t)(l & 1) * 4u;
+            const Ofs o = cel((size_t)(e >> kSrcShift)) * 8u + (uint32_t)(l & 1) * 4u;
             __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)(plane + o),
                                              (void __attribute__((address_space(3))) *)&hbuf[DB ? (row & 1) : 0]
                                                                                            [wvu * NM * 2],
@@ -1369,8 +1377,8 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         }
     };
     if (DMA) {
-        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), (Ofs)(np * 8));
-        if (AL) dma_halo(r0, col_src(r0, -1), col_src(r0, 64), (Ofs)(np * 8));
+        dma_row(r0, col_src(r0, lane >> 1), col_src(r0, 32 + (lane >> 1)), (Ofs)(L.ps * 8));
+        if (AL) dma_halo(r0, col_src(r0, -1), col_src(r0, 64), (Ofs)(L.ps * 8));
         if (wvu == 0) {
             dma_src(r0);
             if (r0 + 1 <= r1) dma_src(r0 + 1);
@@ -1417,7 +1425,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         asm volatile("" : "+v"(ln));
         // likewise the plane stride: the plane addresses are formed per row
         // (scalar arithmetic) instead of being held across the loop
-        Ofs npb = (Ofs)(np * 8);
+        Ofs npb = (Ofs)(L.ps * 8);
         asm volatile("" : "+s"(npb));
         const bool inner = r < r1;
         const size_t k = (size_t)r * A.nx + jc;
@@ -1610,7 +1618,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
                 for (int v = 0; v <= P; ++v) {
                     const double out = base[c][v] - fs[v] * gN[c][v];
-                    plane_at(A.Qout, c * NM + v * NQ, npb)[gofs + (uint32_t)kp] = out;
+                    plane_at(A.Qout, c * NM + v * NQ, npb)[gofs + (Ofs)L.rc(r - 1, jc)] = out;
                     if (v == 0 && g == 0) {
                         mean0[c] = out;
                         if (A.means) A.means[(size_t)c * np + kp] = out;
@@ -1667,7 +1675,7 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int v = 0; v <= P; ++v) qn[c][v] = plane_at(A.Qn, c * NM + v * NQ, npb)[gofs + (uint32_t)k];
+                for (int v = 0; v <= P; ++v) qn[c][v] = plane_at(A.Qn, c * NM + v * NQ, npb)[gofs + (Ofs)L.rc(r, jc)];
         // rhs of row r, all terms but its S face: N face, x faces, then the volume
         // projection accumulated on top by the MFMA
         d4 acc[4];
@@ -1980,25 +1988,25 @@ __device__ __forceinline__ bool zs_mfma_apply(double C[4][4], double gamma, bool
 // false on padding lanes, which compute on a valid cell and store nothing), in
 // the matrix-core layout.  The whole wave must call it.
 template <int P>
-__device__ __forceinline__ void zs_mfma_cells(double *__restrict__ Q, size_t np, double gamma, size_t k, bool live,
+__device__ __forceinline__ void zs_mfma_cells(double *__restrict__ Q, const StLay &L, double gamma, size_t k, bool live,
                                               bool ext)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const bool slot = g <= P;
-    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    const size_t so = (size_t)(slot ? g : 0) * L.ps + L.cell(k);  // this lane's element of plane (c, v)
     double C[4][4];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-            C[c][v] = (v <= P && slot) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+            C[c][v] = (v <= P && slot) ? (Q + (size_t)(c * NM + v * NQ) * L.ps)[so] : 0.0;
     const bool changed = zs_mfma_apply<P>(C, gamma, ext);
     if (live && changed && slot)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] = C[c][v];
+            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * L.ps)[so] = C[c][v];
 }
 
 // TVB limiter then positivity scaling in the matrix-core layout of the fused
@@ -2021,14 +2029,15 @@ __device__ __forceinline__ void zs_mfma_cells(double *__restrict__ Q, size_t np,
 template <int P>
 __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const double *__restrict__ Mn,
                                                const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
-                                               uint8_t *__restrict__ flags, int nx, size_t np, double Mh2,
-                                               double gamma, const uint8_t *__restrict__ zsok, size_t k, bool live,
+                                               uint8_t *__restrict__ flags, int nx, size_t np, const StLay &L,
+                                               double Mh2, double gamma, const uint8_t *__restrict__ zsok, size_t k,
+                                               bool live,
                                                bool lin, bool ext = false)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const bool slot = g <= P;
-    const uint32_t gofs = (uint32_t)((size_t)(slot ? g : 0) * np);
+    const size_t so = (size_t)(slot ? g : 0) * L.ps + L.cell(k);  // this lane's element of plane (c, v)
     double C[4][4];
     // Modes the TVB test reads: (0,0), (1,0) in group 0 and (0,1) in group 1;
     // at P >= 3 every mode (q,0) in group 0 and every mode (0,q) in group q.
@@ -2040,7 +2049,7 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-            C[c][v] = tvb_reads(v) ? (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] : 0.0;
+            C[c][v] = tvb_reads(v) ? (Q + (size_t)(c * NM + v * NQ) * L.ps)[so] : 0.0;
 
     // ---- TVB (cells with bctype 0) ----
     int chg = 0;  // bit c: variable c limited
@@ -2080,7 +2089,7 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            if (!tvb_reads(v) && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k];
+            if (!tvb_reads(v) && v <= P && slot) C[c][v] = (Q + (size_t)(c * NM + v * NQ) * L.ps)[so];
         }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
@@ -2102,15 +2111,15 @@ __device__ __forceinline__ void tvb_mfma_cells(double *__restrict__ Q, const dou
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * np)[gofs + (uint32_t)k] = C[c][v];
+            for (int v = 0; v <= P; ++v) (Q + (size_t)(c * NM + v * NQ) * L.ps)[so] = C[c][v];
 }
 
 template <int P>
 __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q, const double *__restrict__ Mn,
                                                           const int32_t *__restrict__ src,
                                                           const int32_t *__restrict__ bct, uint8_t *__restrict__ flags,
-                                                          int ny, int nx, size_t np, double Mh2, double gamma,
-                                                          const uint8_t *__restrict__ zsok,
+                                                          int ny, int nx, size_t np, StLay L, double Mh2,
+                                                          double gamma, const uint8_t *__restrict__ zsok,
                                                           const StepScalars *__restrict__ S, bool lin, bool ext)
 {
     if (S->done) return;
@@ -2122,7 +2131,7 @@ __global__ __launch_bounds__(256) void dg_tvb_mfma_kernel(double *__restrict__ Q
     const bool live = t0 + col < ncell;
     const size_t t = live ? t0 + col : ncell - 1;
     const size_t k = interior_cell(t, nx, nxu);
-    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, zsok, k, live, lin, ext);
+    tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, L, Mh2, gamma, zsok, k, live, lin, ext);
 }
 
 // Lists the cells the limiter pass must treat, from the byte the fused stage
@@ -2195,8 +2204,8 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
                                                           const uint8_t *__restrict__ upd,
                                                           uint8_t *__restrict__ flags,
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
-                                                          size_t ncell, size_t np, double Mh2, double gamma,
-                                                          const StepScalars *__restrict__ S, RingArgs R,
+                                                          size_t ncell, size_t np, StLay L, double Mh2,
+                                                          double gamma, const StepScalars *__restrict__ S, RingArgs R,
                                                           unsigned nscan, bool lin, bool ext)
 {
     if (S->done) return;  // uniform over the grid
@@ -2224,7 +2233,7 @@ __global__ __launch_bounds__(256) void dg_tvb_scan_kernel(double *__restrict__ Q
         // column col of this group takes listed cell g0 + col (padding repeats the last one)
         const int want = g0 + col < n ? g0 + col : n - 1;
         const size_t kk = (size_t)__shfl((int)k, nth_set_bit(mask, want));
-        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, Mh2, gamma, nullptr, kk, g0 + col < n, lin, ext);
+        tvb_mfma_cells<P>(Q, Mn, src, bct, flags, nx, np, L, Mh2, gamma, nullptr, kk, g0 + col < n, lin, ext);
     }
 }
 
@@ -2249,7 +2258,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
                                                           const uint8_t *__restrict__ upd,
                                                           uint8_t *__restrict__ flags,
                                                           const uint8_t *__restrict__ zsok, int nx, int nxu,
-                                                          size_t ncell, size_t np, double Mh2,
+                                                          size_t ncell, size_t np, StLay L, double Mh2,
                                                           const StepScalars *__restrict__ S, RingArgs R,
                                                           unsigned nscan, unsigned *__restrict__ dcount,
                                                           int32_t *__restrict__ dcells, bool lin)
@@ -2267,7 +2276,10 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
     bool defer = false;
     size_t k = 0;
     if (t < ncell) {
-        k = interior_cell(t, nx, nxu);
+        int ii, jj;
+        interior_rc(t, nxu, ii, jj);
+        k = (size_t)(ii + 3) * nx + (jj + 3);
+        const size_t sk = L.rc(ii + 3, jj + 3);  // the cell in the state arrays
         const unsigned z = zsok[k];
         defer = !(z & 1);
         {
@@ -2281,12 +2293,12 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
                 const int32_t eE = src[k + 1], eW = src[k - 1], eS = src[k + nx], eN = src[k - nx];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const double *q = Q + (size_t)(c * NM) * np + k;
+                    const double *q = Q + (size_t)(c * NM) * L.ps + sk;
                     ub[c] = q[0];
 #pragma unroll
                     for (int d = 1; d <= NT; ++d) {
-                        X[c][d - 1] = q[(size_t)(d * NQ) * np];
-                        Y[c][d - 1] = q[(size_t)d * np];
+                        X[c][d - 1] = q[(size_t)(d * NQ) * L.ps];
+                        Y[c][d - 1] = q[(size_t)d * L.ps];
                     }
                     auto nbm = [&](int32_t e) {
                         double v = Mn[(size_t)c * np + (size_t)(e >> kSrcShift)];
@@ -2316,7 +2328,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
                     if (chg[c])
 #pragma unroll
                         for (int m = 1; m < NM; ++m)
-                            Q[(size_t)(c * NM + m) * np + k] = m == NQ ? axl[c] : (m == 1 ? ayl[c] : 0.0);
+                            Q[(size_t)(c * NM + m) * L.ps + sk] = m == NQ ? axl[c] : (m == 1 ? ayl[c] : 0.0);
             }
             flags[k] = (uint8_t)lim;
         }
@@ -2335,7 +2347,7 @@ __global__ __launch_bounds__(256) void dg_tvb_lane_kernel(double *__restrict__ Q
 // a grid-strided range.  Block 0 clears the next stage's count (the counts
 // rotate over the three stages, so a captured step needs no memset).
 template <int P>
-__global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ Q, size_t np, double gamma,
+__global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ Q, StLay L, double gamma,
                                                            const StepScalars *__restrict__ S,
                                                            unsigned *__restrict__ dcount, int slot,
                                                            const int32_t *__restrict__ dcells, bool ext)
@@ -2347,7 +2359,7 @@ __global__ __launch_bounds__(256) void dg_tvb_defer_kernel(double *__restrict__ 
     const unsigned waves = gridDim.x * (blockDim.x >> 6);
     for (unsigned g0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u; g0 < n; g0 += waves * 16u) {
         const unsigned i = g0 + col < n ? g0 + col : n - 1;  // padding lanes repeat the last cell, store nothing
-        zs_mfma_cells<P>(Q, np, gamma, (size_t)dcells[i], g0 + col < n, ext);
+        zs_mfma_cells<P>(Q, L, gamma, (size_t)dcells[i], g0 + col < n, ext);
     }
 }
 
@@ -2399,7 +2411,7 @@ __global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
 // atomic per wavefront) and counts them in stats[6].
 template <int P>
 __global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__ Q, const int32_t *__restrict__ bct,
-                                                      int nx, int nxu, size_t ncell, size_t np, double Mh2,
+                                                      int nx, int nxu, size_t ncell, StLay L, double Mh2,
                                                       int32_t *__restrict__ cand,
                                                       unsigned long long *__restrict__ stats,
                                                       const StepScalars *__restrict__ S)
@@ -2417,8 +2429,8 @@ __global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__
             for (int c = 0; c < 4; ++c)
 #pragma unroll
                 for (int d = 1; d <= (P >= 3 ? P : 1); ++d) {
-                    C[c][d] = Q[(size_t)(c * NM + d) * np + k];
-                    C[c][d * (P + 1)] = Q[(size_t)(c * NM + d * (P + 1)) * np + k];
+                    C[c][d] = Q[L.at(c * NM + d, k)];
+                    C[c][d * (P + 1)] = Q[L.at(c * NM + d * (P + 1), k)];
                 }
             is = tvb_candidate<P>(C, Mh2);
         }
@@ -2440,7 +2452,7 @@ __global__ __launch_bounds__(256) void dg_cand_kernel(const double *__restrict__
 // Scores the compacted candidates, 16 per wavefront per pass (MFMA), into
 // score[k]; a persistent grid walks the list.
 __global__ __launch_bounds__(256) void dg_score_list_kernel(const double *__restrict__ Q,
-                                                            const int32_t *__restrict__ src, int nx, size_t np,
+                                                            const int32_t *__restrict__ src, int nx, StLay L,
                                                             const int32_t *__restrict__ cand,
                                                             const __bf16 *__restrict__ w1b,
                                                             const __bf16 *__restrict__ w2p, const float *__restrict__ w,
@@ -2453,7 +2465,7 @@ __global__ __launch_bounds__(256) void dg_score_list_kernel(const double *__rest
     for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16; base < n; base += waves * 16) {
         const int t = base + i < n ? base + i : n - 1;
         const size_t k = (size_t)cand[1 + t];
-        const float sc = mlp_score16(Q, src, np, nx, k, w1b, w2p, w);
+        const float sc = mlp_score16(Q, src, L, nx, k, w1b, w2p, w);
         if ((threadIdx.x & 63) < 16 && base + i < n) score[k] = sc;
     }
 }
@@ -2463,20 +2475,21 @@ template <int P>
 __device__ __forceinline__ void learned_cell(double *__restrict__ Q, const double *__restrict__ M,
                                              const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
                                              uint8_t *__restrict__ flags, const float *__restrict__ score, int nx,
-                                             size_t np, double Mh2, double gamma, float lo, float hi,
+                                             size_t np, const StLay &L, double Mh2, double gamma, float lo, float hi,
                                              uint32_t audit_thresh, uint32_t salt,
                                              unsigned long long *__restrict__ stats,
                                              const uint8_t *__restrict__ zsok, size_t k, bool ext)
 {
     constexpr int NQ = P + 1, NM = NQ * NQ;
+    const size_t sk = L.cell(k);  // the cell in the state arrays
     double C[4][NM];
     // the mean and linear modes first: a non-candidate cell whose positivity
     // scaling is certified a no-op (zsok) is left as it is
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        C[c][0] = Q[(size_t)(c * NM) * np + k];
-        C[c][1] = Q[(size_t)(c * NM + 1) * np + k];
-        C[c][NQ] = Q[(size_t)(c * NM + NQ) * np + k];
+        C[c][0] = Q[(size_t)(c * NM) * L.ps + sk];
+        C[c][1] = Q[(size_t)(c * NM + 1) * L.ps + sk];
+        C[c][NQ] = Q[(size_t)(c * NM + NQ) * L.ps + sk];
     }
     const bool cand = bct[k] == 0 && tvb_candidate<P>(C, Mh2);
     if (!cand && zsok && (zsok[k] & 1)) {
@@ -2487,7 +2500,7 @@ __device__ __forceinline__ void learned_cell(double *__restrict__ Q, const doubl
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int m = 0; m < NM; ++m)
-            if (m != 0 && m != 1 && m != NQ) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+            if (m != 0 && m != 1 && m != NQ) C[c][m] = Q[(size_t)(c * NM + m) * L.ps + sk];
     int flag = 0;
     if (cand) {
         const float sc = score[k];
@@ -2534,7 +2547,7 @@ __device__ __forceinline__ void learned_cell(double *__restrict__ Q, const doubl
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * np + k] = C[c][m];
+            for (int m = 0; m < NM; ++m) Q[(size_t)(c * NM + m) * L.ps + sk] = C[c][m];
     }
 }
 
@@ -2550,8 +2563,8 @@ template <int P, bool LIST>
 __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q, const double *__restrict__ M,
                                                          const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
                                                          uint8_t *__restrict__ flags, const float *__restrict__ score,
-                                                         int ny, int nx, size_t np, double Mh2, double gamma, float lo,
-                                                         float hi, uint32_t audit_thresh, uint32_t salt,
+                                                         int ny, int nx, size_t np, StLay L, double Mh2, double gamma,
+                                                         float lo, float hi, uint32_t audit_thresh, uint32_t salt,
                                                          unsigned long long *__restrict__ stats,
                                                          const uint8_t *__restrict__ zsok,
                                                          const int32_t *__restrict__ list,
@@ -2562,7 +2575,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
     const size_t n = LIST ? (size_t)list[0] : (size_t)nxu * nyu;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
         const size_t k = LIST ? (size_t)list[1 + t] : interior_cell(t, nx, nxu);
-        learned_cell<P>(Q, M, src, bct, flags, score, nx, np, Mh2, gamma, lo, hi, audit_thresh, salt, stats,
+        learned_cell<P>(Q, M, src, bct, flags, score, nx, np, L, Mh2, gamma, lo, hi, audit_thresh, salt, stats,
                         LIST ? nullptr : zsok, k, ext);
     }
 }
@@ -2574,7 +2587,7 @@ __global__ __launch_bounds__(128) void dg_learned_kernel(double *__restrict__ Q,
 template <int P>
 __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restrict__ Q, const double *__restrict__ M,
                                                          const int32_t *__restrict__ src, const int32_t *__restrict__ bct,
-                                                         int ny, int nx, size_t np, double Mh2, uint32_t keep_pos,
+                                                         int ny, int nx, size_t np, StLay L, double Mh2, uint32_t keep_pos,
                                                          uint32_t keep_neg, uint32_t salt, __bf16 *__restrict__ X,
                                                          float *__restrict__ y, int *__restrict__ count, int cap,
                                                          const StepScalars *__restrict__ S)
@@ -2589,7 +2602,7 @@ __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restric
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int m = 0; m < NM; ++m) C[c][m] = Q[(size_t)(c * NM + m) * np + k];
+        for (int m = 0; m < NM; ++m) C[c][m] = Q[L.at(c * NM + m, k)];
     double axl[4], ayl[4];
     bool chg[4];
     const bool tv = tvb_test<P>(C, M, src, np, k, nx, Mh2, axl, ayl, chg);
@@ -2600,8 +2613,8 @@ __global__ __launch_bounds__(256) void dg_harvest_kernel(const double *__restric
     const int slot = atomicAdd(count, 1);
     if (slot < 0 || slot >= cap) return;
     double mean[4], scale[4];
-    mlp_scales(Q, np, k, mean, scale);
-    for (int f = 0; f < 192; ++f) X[(size_t)slot * 192 + f] = (__bf16)mlp_feature(Q, src, np, nx, k, mean, scale, f);
+    mlp_scales(Q, L, k, mean, scale);
+    for (int f = 0; f < 192; ++f) X[(size_t)slot * 192 + f] = (__bf16)mlp_feature(Q, src, L, nx, k, mean, scale, f);
     y[slot] = tv ? 1.0f : 0.0f;
 }
 
@@ -2621,6 +2634,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
     A.Mh2 = H.cfg.tvb_M * H.g.da * H.g.da;
     A.ny = H.g.ny; A.nx = H.g.nx; A.nxu = H.g.nx - 6; A.nyu = H.g.ny - 6;
     A.np = H.g.np;
+    A.L = H.lay;
     A.nfx = (size_t)(A.nxu + 1) * A.nyu;
     A.nfy = (size_t)A.nxu * (A.nyu + 1);
     A.da = H.g.da; A.gamma = H.g.gamma;
@@ -2721,7 +2735,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         z0 = H.cand;
         if (A.zsok) z1 = H.tlist;
     }
-    const RingArgs RA{Qn, Qin, Qout, H.src, H.ring, H.nring, H.nm, H.g.np, A.ca, A.cb, stage, A.means};
+    const RingArgs RA{Qn, Qin, Qout, H.src, H.ring, H.nring, H.nm, H.g.np, H.lay, A.ca, A.cb, stage, A.means};
     const unsigned ring_blocks = (unsigned)(((size_t)H.nring * 4 * H.nm + 255) / 256);
     // The ghost-ring combination writes only ring cells of Qout and of the
     // means.  The TVB scan below changes updated cells and reads the means of
@@ -2744,7 +2758,7 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         ProfScope ps(H, "dg_harvest");
         auto frac = [](float f) { return f >= 1.0f ? 0xFFFFFFFFu : (uint32_t)(f * 4294967296.0); };
         hipLaunchKernelGGL((dg_harvest_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout, H.means,
-                           H.src, H.bct, H.g.ny, H.g.nx, H.g.np, Mh2, frac(H.keep_pos), frac(H.keep_neg),
+                           H.src, H.bct, H.g.ny, H.g.nx, H.g.np, H.lay, Mh2, frac(H.keep_pos), frac(H.keep_neg),
                            H.stage_counter, mlp_X(H.harvest), mlp_y(H.harvest), mlp_count(H.harvest),
                            mlp_cap(H.harvest), H.scal);
     }
@@ -2760,11 +2774,11 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
                 ProfScope pl(H, "dg_tvb_lane");
                 hipLaunchKernelGGL((dg_tvb_lane_kernel<P>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256), 0,
                                    H.stream, Qout, H.means, H.src, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell,
-                                   H.g.np, Mh2, H.scal, RA, nscan, H.dcount + stage, H.dcells, H.tvb_linear);
+                                   H.g.np, H.lay, Mh2, H.scal, RA, nscan, H.dcount + stage, H.dcells, H.tvb_linear);
             }
             const unsigned nblk = (unsigned)(H.ncu > 0 ? H.ncu : 256);
             ProfScope pd(H, "dg_tvb_defer");
-            hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.g.np,
+            hipLaunchKernelGGL((dg_tvb_defer_kernel<P>), dim3(nblk), dim3(256), 0, H.stream, Qout, H.lay,
                                H.g.gamma, H.scal, H.dcount, stage, H.dcells, A.zs_ext != 0);
         } else if (A.zsok && !H.no_mfma) {  // scan the fused kernel's marks and limit in place
             // cells per wave: at P = 3 at most two passes of the routine per wave
@@ -2773,15 +2787,15 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
             const unsigned nscan = (unsigned)((ncell + 4 * CW - 1) / (4 * CW));
             hipLaunchKernelGGL((dg_tvb_scan_kernel<P, CW>), dim3(nscan + (fold ? ring_blocks : 0u)), dim3(256),
                                0, H.stream,
-                               Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, Mh2,
-                               H.g.gamma, H.scal, RA, nscan, H.tvb_linear, A.zs_ext != 0);
+                               Qout, H.means, H.src, H.bct, H.upd, H.flags, A.zsok, H.g.nx, A.nxu, ncell, H.g.np, H.lay,
+                               Mh2, H.g.gamma, H.scal, RA, nscan, H.tvb_linear, A.zs_ext != 0);
         } else if (H.no_mfma || P < 3)
             hipLaunchKernelGGL((dg_tvb_kernel<P>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout, H.means,
-                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, H.scal, H.tvb_linear,
+                               H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.lay, Mh2, H.g.gamma, H.scal, H.tvb_linear,
                                A.zs_ext != 0);
         else
             hipLaunchKernelGGL((dg_tvb_mfma_kernel<P>), dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, H.stream,
-                               Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma, A.zsok,
+                               Qout, H.means, H.src, H.bct, H.flags, H.g.ny, H.g.nx, H.g.np, H.lay, Mh2, H.g.gamma, A.zsok,
                                H.scal, H.tvb_linear, A.zs_ext != 0);
     }
     if (ncell > 0 && H.cfg.limiter == ZK_LIMITER_LEARNED && P == 2) {
@@ -2793,14 +2807,14 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
                                    A.zsok, H.upd, H.flags, H.g.nx, A.nxu, ncell, H.tlist, H.cand, H.stats + 6, H.scal);
             } else {
                 hipLaunchKernelGGL((dg_cand_kernel<P>), dim3((ncell + 255) / 256), dim3(256), 0, H.stream, Qout,
-                                   H.bct, H.g.nx, A.nxu, ncell, H.g.np, Mh2, H.cand, H.stats, H.scal);
+                                   H.bct, H.g.nx, A.nxu, ncell, H.lay, Mh2, H.cand, H.stats, H.scal);
             }
         }
         {
             ProfScope ps(H, "dg_score");
             const size_t want = (ncell + 63) / 64, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 8;
             hipLaunchKernelGGL(dg_score_list_kernel, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0,
-                               H.stream, Qout, H.src, H.g.nx, H.g.np, H.cand, mlp_w1b(H.mlp), mlp_w2p(H.mlp),
+                               H.stream, Qout, H.src, H.g.nx, H.lay, H.cand, mlp_w1b(H.mlp), mlp_w2p(H.mlp),
                                mlp_w(H.mlp), H.score, H.scal);
         }
         ProfScope ps(H, "dg_learned");
@@ -2808,13 +2822,13 @@ void dg_stage_p(Handle &H, const double *Qn, const double *Qin, double *Qout, in
         if (A.zsok) {
             const size_t want = (ncell + 127) / 128, cap = (size_t)(H.ncu > 0 ? H.ncu : 256) * 16;
             hipLaunchKernelGGL((dg_learned_kernel<P, true>), dim3((unsigned)(want < cap ? want : cap)), dim3(128), 0,
-                               H.stream, Qout, H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2,
-                               H.g.gamma, H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist,
+                               H.stream, Qout, H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, H.lay,
+                               Mh2, H.g.gamma, H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist,
                                H.scal, A.zs_ext != 0);
         } else {
             hipLaunchKernelGGL((dg_learned_kernel<P, false>), dim3((ncell + 127) / 128), dim3(128), 0, H.stream, Qout,
-                               H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, Mh2, H.g.gamma,
-                               H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist, H.scal,
+                               H.means, H.src, H.bct, H.flags, H.score, H.g.ny, H.g.nx, H.g.np, H.lay, Mh2,
+                               H.g.gamma, H.mlp_lo, H.mlp_hi, audit, H.stage_counter, H.stats, A.zsok, H.tlist, H.scal,
                                A.zs_ext != 0);
         }
     }
@@ -2878,15 +2892,15 @@ void dg_max_speed(Handle &H, const double *Q)
     if (blocks > 2048) blocks = 2048;
     switch (H.cfg.P) {
     case 1:
-        hipLaunchKernelGGL((dg_max_speed_kernel<1>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+        hipLaunchKernelGGL((dg_max_speed_kernel<1>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np, H.lay,
                            H.g.gamma, H.lam_bits);
         break;
     case 2:
-        hipLaunchKernelGGL((dg_max_speed_kernel<2>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+        hipLaunchKernelGGL((dg_max_speed_kernel<2>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np, H.lay,
                            H.g.gamma, H.lam_bits);
         break;
     default:
-        hipLaunchKernelGGL((dg_max_speed_kernel<3>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np,
+        hipLaunchKernelGGL((dg_max_speed_kernel<3>), dim3(blocks), dim3(256), 0, H.stream, Q, H.bct, H.g.np, H.lay,
                            H.g.gamma, H.lam_bits);
         break;
     }

@@ -25,25 +25,25 @@ namespace zk {
 
 // Feature k of the cell whose own stage-output modes are Q (planes c*9+m) at
 // cell k0: k = n*36 + c*9 + m, n = self, E (j+1), W (j-1), S (i+1), N (i-1).
-__device__ __forceinline__ float mlp_feature(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np,
-                                             int nx, size_t k0, const double mean[4], const double scale[4], int k)
+__device__ __forceinline__ float mlp_feature(const double *__restrict__ Q, const int32_t *__restrict__ src,
+                                             const StLay &L, int nx, size_t k0, const double mean[4], const double scale[4], int k)
 {
     if (k >= MLP_FEAT) return 0.0f;
     const int n = k / 36, c = (k / 9) % 4, m = k % 9;
     const long off[5] = {0, 1, -1, (long)nx, -(long)nx};
     const size_t cell = (size_t)((long)k0 + off[n]);
     const int32_t e = src[cell];
-    double v = Q[(size_t)(c * 9 + m) * np + (size_t)(e >> kSrcShift)];
+    double v = Q[L.at(c * 9 + m, (size_t)(e >> kSrcShift))];
     if ((c == 1 && (e & 1)) || (c == 2 && (e & 2))) v = -v;
     if (m == 0) v -= mean[c];
     return (float)(v / scale[c]);
 }
 
-__device__ __forceinline__ void mlp_scales(const double *__restrict__ Q, size_t np, size_t k, double mean[4],
+__device__ __forceinline__ void mlp_scales(const double *__restrict__ Q, const StLay &L, size_t k, double mean[4],
                                            double scale[4])
 {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) mean[c] = Q[(size_t)(c * 9) * np + k];
+    for (int c = 0; c < 4; ++c) mean[c] = Q[L.at(c * 9, k)];
     const double r = fabs(mean[0]);
 #pragma unroll
     for (int c = 0; c < 4; ++c) scale[c] = fabs(mean[c]) + r + 1e-12;
@@ -52,14 +52,14 @@ __device__ __forceinline__ void mlp_scales(const double *__restrict__ Q, size_t 
 // Score sigmoid(MLP(features)) of 16 cells per wave.  cells[t] is the grid
 // index of work item t; the wave must be fully populated (padded lanes reuse a
 // valid cell).  Returns the score of the lane's cell (column i) in every lane.
-__device__ __forceinline__ float mlp_score16(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np, int nx,
+__device__ __forceinline__ float mlp_score16(const double *__restrict__ Q, const int32_t *__restrict__ src, const StLay &L, int nx,
                              size_t k0, const __bf16 *__restrict__ w1b, const __bf16 *__restrict__ w2p,
                              const float *__restrict__ w)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15;
     (void)i;
     double mean[4], scale[4];
-    mlp_scales(Q, np, k0, mean, scale);
+    mlp_scales(Q, L, k0, mean, scale);
     // layer 1: H1[h][cell] = sum_k W1[h][k] X[cell][k]; B operand = this lane's features
     f4 h1acc[4];
 #pragma unroll
@@ -68,7 +68,7 @@ __device__ __forceinline__ float mlp_score16(const double *__restrict__ Q, const
     for (int s = 0; s < MLP_IN / 32; ++s) {
         bf16x8 xv;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xv[e] = (__bf16)mlp_feature(Q, src, np, nx, k0, mean, scale, 32 * s + 8 * g + e);
+        for (int e = 0; e < 8; ++e) xv[e] = (__bf16)mlp_feature(Q, src, L, nx, k0, mean, scale, 32 * s + 8 * g + e);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const bf16x8 av = *reinterpret_cast<const bf16x8 *>(w1b + (size_t)(16 * t + (lane & 15)) * MLP_IN + 32 * s + 8 * g);

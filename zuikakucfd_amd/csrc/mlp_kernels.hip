@@ -224,20 +224,20 @@ __global__ void zero_f_kernel(float *p, int n)
 namespace {
 
 // Features of a list of cells (for tests and for harvesting): X[t][k].
-__global__ void features_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src, size_t np, int nx,
+__global__ void features_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src, zk::StLay L, int nx,
                                 const int32_t *__restrict__ cells, int n, __bf16 *__restrict__ X)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * MLP_IN) return;
     const int c = t / MLP_IN, k = t % MLP_IN;
     double mean[4], scale[4];
-    zk::mlp_scales(Q, np, (size_t)cells[c], mean, scale);
-    X[t] = (__bf16)zk::mlp_feature(Q, src, np, nx, (size_t)cells[c], mean, scale, k);
+    zk::mlp_scales(Q, L, (size_t)cells[c], mean, scale);
+    X[t] = (__bf16)zk::mlp_feature(Q, src, L, nx, (size_t)cells[c], mean, scale, k);
 }
 
 // Scores of a list of cells through the fused path (16 cells per wave).
 __global__ __launch_bounds__(256) void score_kernel(const double *__restrict__ Q, const int32_t *__restrict__ src,
-                                                    size_t np, int nx, const int32_t *__restrict__ cells, int n,
+                                                    zk::StLay L, int nx, const int32_t *__restrict__ cells, int n,
                                                     const __bf16 *__restrict__ w1b, const __bf16 *__restrict__ w2p,
                                                     const float *__restrict__ w, float *__restrict__ out)
 {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void score_kernel(const double *__restrict__ Q
     if (base >= n) return;
     const int i = threadIdx.x & 15;
     const int t = base + i < n ? base + i : n - 1;
-    const float sc = zk::mlp_score16(Q, src, np, nx, (size_t)cells[t], w1b, w2p, w);
+    const float sc = zk::mlp_score16(Q, src, L, nx, (size_t)cells[t], w1b, w2p, w);
     if ((threadIdx.x & 63) < 16 && base + i < n) out[base + i] = sc;
 }
 
@@ -595,10 +595,10 @@ zk_status mlp_features_of(Handle &H, zk_mlp *M, const int32_t *cells, int n, uin
         return ZK_ERR_HIP;
     }
     (void)hipMemcpyAsync(dc, cells, (size_t)n * 4, hipMemcpyHostToDevice, H.stream);
-    hipLaunchKernelGGL(features_kernel, dim3((n * MLP_IN + 255) / 256), dim3(256), 0, H.stream, H.q, H.src, H.g.np,
+    hipLaunchKernelGGL(features_kernel, dim3((n * MLP_IN + 255) / 256), dim3(256), 0, H.stream, H.q, H.src, H.lay,
                        H.g.nx, dc, n, dx);
     if (M)
-        hipLaunchKernelGGL(score_kernel, dim3(((n + 15) / 16 + 3) / 4), dim3(256), 0, H.stream, H.q, H.src, H.g.np,
+        hipLaunchKernelGGL(score_kernel, dim3(((n + 15) / 16 + 3) / 4), dim3(256), 0, H.stream, H.q, H.src, H.lay,
                            H.g.nx, dc, n, M->w1b, M->w2p, M->w, ds);
     if (X) (void)hipMemcpyAsync(X, dx, (size_t)n * MLP_IN * 2, hipMemcpyDeviceToHost, H.stream);
     if (M && scores) (void)hipMemcpyAsync(scores, ds, (size_t)n * 4, hipMemcpyDeviceToHost, H.stream);

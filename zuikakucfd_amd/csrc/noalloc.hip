@@ -140,6 +140,9 @@ zk_status zk_weno_step_noalloc(const zk_config *cfg, int ny, int nx, double da, 
     H.cfg = *cfg;
     H.nm = 1;
     H.g.ny = ny; H.g.nx = nx; H.g.np = (size_t)ny * nx; H.g.da = da; H.g.gamma = gamma;
+    H.lay.ps = H.g.np;  // plane after plane
+    H.lay.nx = (uint32_t)nx;
+    H.lay.rs = 0;
     H.stream = static_cast<hipStream_t>(stream);
     H.loaded = true;
     H.scal = reinterpret_cast<zk::StepScalars *>(base);

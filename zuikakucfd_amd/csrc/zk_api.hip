@@ -90,14 +90,14 @@ __global__ void cfl_dt_kernel(zk::StepScalars *S, const unsigned long long *lam_
 // cell): records the smallest cell*4 + variable found and the step count
 // `step` in health[0], health[1] (atomic minima; the record starts at all
 // ones).  Reads 32 bytes per cell, a fraction of one stage's traffic.
-__global__ __launch_bounds__(256) void health_kernel(const double *__restrict__ Q, size_t np, int nm,
+__global__ __launch_bounds__(256) void health_kernel(const double *__restrict__ Q, size_t np, zk::StLay L, int nm,
                                                      unsigned long long *health, unsigned long long step)
 {
     unsigned long long key = ~0ull;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (size_t)gridDim.x * blockDim.x)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double x = Q[(size_t)c * nm * np + k];
+            const double x = Q[L.at(c * nm, k)];
             if (!(fabs(x) <= 1.79769313486231570815e308)) {
                 const unsigned long long kk = (unsigned long long)k * 4 + c;
                 if (kk < key) key = kk;
@@ -109,11 +109,36 @@ __global__ __launch_bounds__(256) void health_kernel(const double *__restrict__ 
     }
 }
 
+// Copies between a state array in layout L and plane-major planes of np cells:
+// plane i of the plane-major side is state plane i * pstep (blockIdx.y = i).
+// to_state: pm -> state, else state -> pm.
+__global__ __launch_bounds__(256) void relayout_kernel(double *__restrict__ state, double *__restrict__ pm, size_t np,
+                                                       zk::StLay L, int pstep, bool to_state)
+{
+    const int i = blockIdx.y;
+    double *plane = pm + (size_t)i * np;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t s = L.at(i * pstep, k);
+        if (to_state) state[s] = plane[k];
+        else plane[k] = state[s];
+    }
+}
+
+// Plane-major planes 0, pstep, 2 pstep, ... (n of them) of the state q to or
+// from pm, on the handle's stream.
+void relayout(Handle *H, double *pm, int n, int pstep, bool to_state)
+{
+    unsigned bx = (unsigned)((H->g.np + 255) / 256);
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(relayout_kernel, dim3(bx, (unsigned)n), dim3(256), 0, H->stream, H->q, pm, H->g.np, H->lay,
+                       pstep, to_state);
+}
+
 void enqueue_health(Handle *H)
 {
     int blocks = (int)((H->g.np + 255) / 256);
     if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(health_kernel, dim3(blocks), dim3(256), 0, H->stream, H->q, H->g.np, H->nm, H->health,
+    hipLaunchKernelGGL(health_kernel, dim3(blocks), dim3(256), 0, H->stream, H->q, H->g.np, H->lay, H->nm, H->health,
                        (unsigned long long)H->steps_enqueued);
 }
 
@@ -414,12 +439,15 @@ zk_status zk_create(const zk_config *cfg, zk_handle **out)
     // the fused DG kernel's workgroup form), ZK_MF4=0 (its 16x16x4 products instead of the
     // 4x4x4 blocks), ZK_TVB_LANE=0 (the 16-cell TVB scan instead of one lane per cell),
     // ZK_TUNE=<dg_variant> (a value zk_set_tuning accepts; anything else is ignored),
-    // ZK_FUSED_GENERIC=1 (the fused kernel form used for state arrays of 4 GiB and more)
+    // ZK_FUSED_GENERIC=1 (the fused kernel form used for state arrays of 4 GiB and more),
+    // ZK_FUSED_AL=0 (the fused kernel's 62-column runs at every stage), ZK_DG_PLANES=1 (DG
+    // state planes one after the other instead of the planes of each row side by side)
     if (const char *v = std::getenv("ZK_NO_MFMA")) H->no_mfma = v[0] == '1';
     if (const char *v = std::getenv("ZK_FUSED_GENERIC")) H->fused_generic = v[0] == '1';
     if (const char *v = std::getenv("ZK_MF4")) H->mf4 = v[0] != '0';
     if (const char *v = std::getenv("ZK_TVB_LANE")) H->tvb_lane = v[0] != '0';
     if (const char *v = std::getenv("ZK_FUSED_AL")) H->fused_al = v[0] != '0';
+    if (const char *v = std::getenv("ZK_DG_PLANES")) H->dg_planes = v[0] == '1';
     if (const char *v = std::getenv("ZK_DMA"))
         if (v[0] >= '0' && v[0] <= '3' && v[1] == 0) H->dma = v[0] - '0';
     if (const char *v = std::getenv("ZK_TUNE")) {
@@ -481,15 +509,22 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma, const d
     free_all(H);
     H->g.ny = ny; H->g.nx = nx; H->g.np = (size_t)ny * nx; H->g.da = da; H->g.gamma = gamma;
     const size_t np = H->g.np, nst = 4 * (size_t)H->nm * np;
+    // DG: the 4 nm planes of each grid row side by side ([row][plane][column]);
+    // WENO (and DG under ZK_DG_PLANES=1): plane after plane
+    zk::StLay &L = H->lay;
+    const bool rows = H->cfg.method == ZK_METHOD_DG && !H->dg_planes;
+    L.ps = rows ? (size_t)nx : np;
+    L.nx = (uint32_t)nx;
+    L.rs = rows ? (uint32_t)((4 * H->nm - 1) * nx) : 0u;
 
     // conserved variables, src_WENO/fluid.cpp:86-93 (DG modes >= 1 are zero)
     std::vector<double> Q(nst, 0.0);
     for (size_t k = 0; k < np; ++k) {
         const double r = rho[k], ux = u[k], vy = v[k], pr = p[k];
-        Q[k] = r;
-        Q[(size_t)H->nm * np + k] = r * ux;
-        Q[2 * (size_t)H->nm * np + k] = r * vy;
-        Q[3 * (size_t)H->nm * np + k] = pr / (gamma - 1.0) + 0.5 * r * (ux * ux + vy * vy);
+        Q[L.at(0, k)] = r;
+        Q[L.at(H->nm, k)] = r * ux;
+        Q[L.at(2 * H->nm, k)] = r * vy;
+        Q[L.at(3 * H->nm, k)] = pr / (gamma - 1.0) + 0.5 * r * (ux * ux + vy * vy);
     }
     H->h_bct.assign(bctype, bctype + np);
     std::vector<int32_t> src;
@@ -605,8 +640,16 @@ zk_status zk_set_state(zk_handle *h, const double *Q)
         return ZK_ERR_ARG;
     }
     ZK_HIP(hipSetDevice(H->cfg.device));
-    ZK_HIP(hipMemcpyAsync(H->q, Q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyHostToDevice,
-                          H->stream));
+    if (H->lay.rs == 0) {
+        ZK_HIP(hipMemcpyAsync(H->q, Q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyHostToDevice,
+                              H->stream));
+    } else {
+        // through stage buffer A (rewritten by every stage before it is read)
+        ZK_HIP(hipMemcpyAsync(H->qa, Q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyHostToDevice,
+                              H->stream));
+        relayout(H, H->qa, 4 * H->nm, 1, true);
+        ZK_HIP(hipGetLastError());
+    }
     // a new state starts a new record
     ZK_HIP(hipMemsetAsync(H->health, 0xff, 2 * sizeof(unsigned long long), H->stream));
     H->numeric_failed = false;
@@ -624,7 +667,14 @@ zk_status zk_get_state(zk_handle *h, double *Q)
         return ZK_ERR_ARG;
     }
     ZK_HIP(hipSetDevice(H->cfg.device));
-    ZK_HIP(hipMemcpyAsync(Q, H->q, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyDeviceToHost,
+    const double *from = H->q;
+    if (H->lay.rs != 0) {
+        // plane-major copy in stage buffer A (rewritten by every stage before it is read)
+        relayout(H, H->qa, 4 * H->nm, 1, false);
+        ZK_HIP(hipGetLastError());
+        from = H->qa;
+    }
+    ZK_HIP(hipMemcpyAsync(Q, from, 4 * (size_t)H->nm * H->g.np * sizeof(double), hipMemcpyDeviceToHost,
                           H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
     return check_health(H);
@@ -636,16 +686,25 @@ zk_status zk_get_conserved(zk_handle *h, double *U0, double *U1, double *U2, dou
     zk_status st = check_args(H);
     if (st) return st;
     double *dst[4] = {U0, U1, U2, U3};
-    ZK_HIP(hipSetDevice(H->cfg.device));
-    ZK_HIP(hipStreamSynchronize(H->stream));
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < 4; ++c)
         if (!dst[c]) {
             H->err = "zk_get_conserved: null array";
             return ZK_ERR_ARG;
         }
-        ZK_HIP(hipMemcpyAsync(dst[c], H->q + (size_t)c * H->nm * H->g.np, H->g.np * sizeof(double),
-                              hipMemcpyDeviceToHost, H->stream));
+    ZK_HIP(hipSetDevice(H->cfg.device));
+    ZK_HIP(hipStreamSynchronize(H->stream));
+    const double *from = H->q;
+    size_t pstride = (size_t)H->nm * H->g.np;
+    if (H->lay.rs != 0) {
+        // the four mean planes, plane-major in stage buffer A
+        relayout(H, H->qa, 4, H->nm, false);
+        ZK_HIP(hipGetLastError());
+        from = H->qa;
+        pstride = H->g.np;
     }
+    for (int c = 0; c < 4; ++c)
+        ZK_HIP(hipMemcpyAsync(dst[c], from + (size_t)c * pstride, H->g.np * sizeof(double), hipMemcpyDeviceToHost,
+                              H->stream));
     ZK_HIP(hipStreamSynchronize(H->stream));
     return check_health(H);
 }

@@ -21,6 +21,25 @@ namespace zk {
 constexpr int kSrcShift = 3;
 constexpr int32_t kSrcUpdated = 4;
 
+// Layout of a state array: element (plane p, cell k) sits at
+// p * ps + k + (k / nx) * rs.  WENO keeps its planes one after the other
+// (ps = ny nx, rs = 0); DG puts the planes of a grid row side by side,
+// [row][plane][column] (ps = nx, rs = (planes - 1) nx), so that the row of
+// every plane a stage kernel reads or writes lies in one span of memory
+// (fewer DRAM pages and translations per row than planes ny nx apart).
+struct StLay {
+    size_t ps = 0;
+    uint32_t nx = 1, rs = 0;
+    // offset of plane 0 of cell k (cell indices stay below 2^31)
+    __host__ __device__ __forceinline__ size_t cell(size_t k) const
+    {
+        return k + (size_t)((uint32_t)k / nx) * rs;
+    }
+    __host__ __device__ __forceinline__ size_t at(int p, size_t k) const { return (size_t)p * ps + cell(k); }
+    // offset of plane 0 of the cell in row i, column j (no division)
+    __host__ __device__ __forceinline__ size_t rc(size_t i, size_t j) const { return i * (nx + rs) + j; }
+};
+
 // Per-step scalars that stage kernels read from device memory, so that a
 // captured graph replays with a dt the device itself computed.
 struct StepScalars {
@@ -58,6 +77,8 @@ struct Handle {
     bool mf4 = true;          // fused DG kernel (two row buffers) on 4x4x4 matrix-core blocks (ZK_MF4=0: 16x16x4)
     bool fused_generic = false;  // ZK_FUSED_GENERIC=1: the fused kernel's form for state arrays of 4 GiB and more
                                  // (stage read at run time, 64-bit plane offsets) at every size
+    StLay lay;                // layout of q, qa, qb
+    bool dg_planes = false;   // DG state planes one after the other (ZK_DG_PLANES=1) instead of rows interleaved
     bool fused_al = true;     // fused DG kernel on aligned 64-column blocks (ZK_FUSED_AL=0: the 62-column runs)
     bool tvb_lane = true;     // TVB after the fused kernel: one lane per cell (ZK_TVB_LANE=0: the 16-cell scan)
     bool tvb_linear = false;  // ZK_LIMITER_TVB_LINEAR: the TVB test on the linear modes alone at every P
@@ -70,7 +91,7 @@ struct Handle {
     int64_t n_updated = 0;
 
     // device buffers
-    double *q = nullptr;      // state U^n (4*nm planes)
+    double *q = nullptr;      // state U^n (4*nm planes, laid out as lay says)
     double *qa = nullptr;     // stage buffer A
     double *qb = nullptr;     // stage buffer B
     int32_t *src = nullptr;   // ghost-fill map (ny*nx)
