@@ -1496,6 +1496,49 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             // read it; with two buffers the next row goes to the other one,
             // last read in the previous iteration (before this row's barrier)
             if (!DB) __syncthreads();
+            if (AL && inner && wvu == 1) {
+                // The two edge fluxes of row r, from LDS alone, before the next
+                // row's loads to LDS go out (the compiler makes an LDS read after
+                // those loads wait for all of them to land: 2% of the kernel's
+                // time at 1024^2 P = 3).  Traces: lanes (g, col) of block
+                // b = col / 4 hold variable col % 4 of the E trace of the left
+                // side column (b = 0), the W trace of the right one (b = 1), the
+                // W trace of column 0 (b = 2) and the E trace of column 63
+                // (b = 3), each at point g, with the rows' trace products.
+                int l2 = lane;
+                asm volatile("" : "+v"(l2));
+                const int g2 = l2 >> 4, bk = (l2 >> 2) & 3, cv = l2 & 3, gq = g2 <= P ? g2 : 0;
+                const int32_t eh = bk < 2 ? srcbuf[r % NS][64 + bk] : srcbuf[r % NS][bk == 2 ? 0 : 63];
+                const bool neg = (cv == 1 && (eh & 1)) || (cv == 2 && (eh & 2));
+                auto qb = qbuf[r & 1];
+                double tr = 0.0;
+#pragma unroll
+                for (int v = 0; v <= P; ++v) {
+                    const int pl = cv * NM + v * NQ + gq;
+                    double q = g2 > P ? 0.0 : bk < 2 ? hbuf[r & 1][pl * 2 + bk] : qb[pl][bk == 2 ? 0 : 63];
+                    q = neg ? -q : q;
+                    const double t = mf4(M4.aL[l2], q);
+                    // W traces (blocks 1, 2) alternate the sign of the odd modes
+                    tr = v == 0 ? t : tr + (((bk == 1 || bk == 2) && (v & 1)) ? -t : t);
+                }
+                lds_ht[bk][g2][cv] = tr;
+                // lanes (g, 0): the left edge face at point g (side column | column 0);
+                // lanes (g, 1): the right one (column 63 | side column)
+                const int c2 = l2 & 15;
+                double Lx[4], Rx[4], Fx[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    Lx[c] = lds_ht[c2 == 0 ? 0 : 3][g2][c];
+                    Rx[c] = lds_ht[c2 == 0 ? 2 : 1][g2][c];
+                }
+                if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
+                else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double gb = mf4(M4.aWL[l2], Fx[c]);  // onto py = g
+                    if (c2 < 2) lds_bf[c2][g2][c] = gb;
+                }
+            }
             if (inner) {
                 const int32_t *sn = srcbuf[(r + 1) % NS];
                 // one column per lane: is row r + 1 its own source throughout?
@@ -1513,46 +1556,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
         } else {
             load(r, Q);
             upd = A.upd[k] != 0;
-        }
-        if (AL && inner && wvu == 1) {
-            // The two edge fluxes of row r, from LDS alone.  Traces: lanes
-            // (g, col) of block b = col / 4 hold variable col % 4 of the E trace
-            // of the left side column (b = 0), the W trace of the right one
-            // (b = 1), the W trace of column 0 (b = 2) and the E trace of
-            // column 63 (b = 3), each at point g, with the rows' trace products.
-            int l2 = lane;
-            asm volatile("" : "+v"(l2));
-            const int g2 = l2 >> 4, bk = (l2 >> 2) & 3, cv = l2 & 3, gq = g2 <= P ? g2 : 0;
-            const int32_t eh = bk < 2 ? srcbuf[r % NS][64 + bk] : srcbuf[r % NS][bk == 2 ? 0 : 63];
-            const bool neg = (cv == 1 && (eh & 1)) || (cv == 2 && (eh & 2));
-            auto qb = qbuf[r & 1];
-            double tr = 0.0;
-#pragma unroll
-            for (int v = 0; v <= P; ++v) {
-                const int pl = cv * NM + v * NQ + gq;
-                double q = g2 > P ? 0.0 : bk < 2 ? hbuf[r & 1][pl * 2 + bk] : qb[pl][bk == 2 ? 0 : 63];
-                q = neg ? -q : q;
-                const double t = mf4(M4.aL[l2], q);
-                // W traces (blocks 1, 2) alternate the sign of the odd modes
-                tr = v == 0 ? t : tr + (((bk == 1 || bk == 2) && (v & 1)) ? -t : t);
-            }
-            lds_ht[bk][g2][cv] = tr;
-            // lanes (g, 0): the left edge face at point g (side column | column 0);
-            // lanes (g, 1): the right one (column 63 | side column)
-            const int c2 = l2 & 15;
-            double Lx[4], Rx[4], Fx[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                Lx[c] = lds_ht[c2 == 0 ? 0 : 3][g2][c];
-                Rx[c] = lds_ht[c2 == 0 ? 2 : 1][g2][c];
-            }
-            if (A.flux == ZK_FLUX_HLL) hll_f64(Lx, Rx, gam, Fx);
-            else hllc_dg_fast_f64(Lx, Rx, gam, Fx);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double gb = mf4(M4.aWL[l2], Fx[c]);  // onto py = g
-                if (c2 < 2) lds_bf[c2][g2][c] = gb;
-            }
         }
         // face traces of this row: element side (W, E, N, S) at quadrature point g
         d4 Tr[4];
@@ -1667,9 +1670,6 @@ __global__ __launch_bounds__(256, MINW) void dg_fused_kernel(DgArgs A, int R, in
             }
         }
         if (!inner) break;
-        // U^n of row r, needed at the end of this iteration.  In the first
-        // stage U^n is the row just read (an updated cell's ghost-map entry is
-        // itself, unsigned), so only the cells that are not updated read it.
         double qn[4][4];
         if (store_col && slot && !(first && upd))
 #pragma unroll
