@@ -594,8 +594,8 @@ def main() -> int:
         # what the counters of this kernel show (committed rocprofv3 summaries)
         line["roofline"]["measured_limit"] = (
             "memory access pattern and FP64 issue together, not HBM bandwidth: with every global access kept in "
-            "the L2 the same instruction stream runs at about 306 us per launch against about 385 measured; the "
-            "waves issue 31-36% of their cycles, wait to issue 29-42% and wait on memory or barriers 21-40% "
+            "the L2 the same instruction stream runs at about 306 us per launch against about 376 measured; the "
+            "waves issue 33-37% of their cycles, wait to issue 38-42% and wait on memory or barriers 22-30% "
             "(profiles/dg3_tvb_riemann1024_stall_pmc.txt)")
     if args.resume_state:
         line["resumed"] = ("the timed window started from the state a --save-state run reached at step "
