@@ -107,11 +107,14 @@ zk_status zk_load(zk_handle *h, int ny, int nx, double da, double gamma,
                   const double *rho, const double *u, const double *v,
                   const double *p, const int32_t *bctype);
 
-/* Overwrite the whole state (4*nm planes), e.g. L2-projected DG modes or a
- * restore.  No reference counterpart (the reference cannot resume). */
+/* Overwrite the whole state (4*nm planes, plane c*nm + m of variable c and
+ * mode m, each ny*nx row-major), e.g. L2-projected DG modes or a restore.
+ * No reference counterpart (the reference cannot resume).  A DG state lives
+ * on the device with the planes of each grid row side by side; the call
+ * converts on the device, through the first stage buffer. */
 zk_status zk_set_state(zk_handle *h, const double *Q);
 
-/* Copy the whole state (4*nm planes) to the host. */
+/* Copy the whole state (4*nm planes, as zk_set_state takes them) to the host. */
 zk_status zk_get_state(zk_handle *h, double *Q);
 
 /* Copy the cell averages U0..U3 (mode 0) into four caller-owned planes.
