@@ -444,6 +444,25 @@ def device_copy(nbytes: int) -> int:
     return nbytes
 
 
+def stream_reference(state_bytes: int) -> dict:
+    """Streaming bandwidth this GPU gives for the stage's read/write mixes, over
+    arrays the size of one state array (zk_stream_bandwidth): a copy (1 read +
+    1 write: the first RK stage reads U^n and writes U^1) and an add (2 reads +
+    1 write: the later stages).  `stage_mix_gbs` is the rate at which those
+    kernels would move one launch's algorithmic bytes (the three stages'
+    average): the practical ceiling of a stage kernel that moves exactly those
+    bytes."""
+    from zuikakucfd_amd import zk
+
+    copy, add = zk.stream_bandwidth(state_bytes)
+    t = (2 * state_bytes / copy + 2 * 3 * state_bytes / add) / 3 / 1e9  # seconds per launch, stage average
+    return {"copy_gbs": copy, "add_gbs": add, "stage_mix_gbs": (8 * state_bytes / 3) / t / 1e9,
+            "array_bytes": state_bytes,
+            "note": "zk_stream_bandwidth: a copy (1 read + 1 write) and an add (2 reads + 1 write) over arrays of one "
+                    "state's size, 16-byte accesses, best of 5 after a first run; stage_mix_gbs weights them as the "
+                    "3 RK stages move their bytes"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +486,8 @@ def main() -> int:
     ap.add_argument("--resume-state", default=None, metavar="NPY",
                     help="start the timed window from a state written by --save-state for the same workload and "
                          "warm-up, instead of advancing to it (short runs under a profiler)")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the streaming-bandwidth reference measured beside the roofline")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB on the device (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -548,6 +569,7 @@ def main() -> int:
     s.sync()
     load_ms = (time.perf_counter() - t0) * 1e3
     s.close()
+    stream = None if args.no_stream else stream_reference(4 * nm * m.ny * m.nx * 8)
     boundary = {"load_ms": load_ms, "get_conserved_ms": min(get_ms),
                 "cell_stage_per_s_with_output_every_100_steps":
                     100 * 3 * nupd / ((100 * ms / args.steps + min(get_ms)) / 1e3),
@@ -581,6 +603,7 @@ def main() -> int:
                      "kernel_avg_ms": avg_ms, "algo_bytes_per_cell_stage": bps,
                      "algo_bytes_per_launch": bps * nupd},
         "stage_hbm_frac": rate * bps / (HBM_PEAK_GBS * 1e9),
+        "stream_reference": stream,
         "median_of_5": {"value": rate_median5, "ms_per_step": reps[2] / args.steps,
                         "note": "the timed block and four more right after it (HIP events)"},
         "early_window": early,
@@ -590,6 +613,9 @@ def main() -> int:
         "last_step": last_step,
         "boundary": boundary,
     }
+    if stream:
+        # the same bytes moved by plain streaming kernels: the practical ceiling
+        line["roofline"]["frac_of_stream_mix"] = achieved / stream["stage_mix_gbs"]
     if args.workload.startswith("dg3_tvb"):
         # what the counters of this kernel show (committed rocprofv3 summaries)
         line["roofline"]["measured_limit"] = (

@@ -165,6 +165,14 @@ zk_status zk_profile_report(zk_handle *h, char *buf, size_t len);
  * the loop domain of src_WENO/fluid.cpp:398-402). */
 zk_status zk_updated_cells(zk_handle *h, int64_t *out);
 
+/* Streaming bandwidth of GPU `device` for the read/write mixes of a stage
+ * kernel (measurement helper, no reference counterpart): a copy (1 read +
+ * 1 write) and an add (2 reads + 1 write) over three arrays of array_bytes
+ * each, 16-byte accesses, best of five timed runs after a first one, in GB/s.
+ * The practical ceiling a stage's roofline fraction is read against.  It
+ * allocates and frees the arrays; errors: zk_last_error(NULL). */
+zk_status zk_stream_bandwidth(int device, size_t array_bytes, double *copy_gbs, double *add_gbs);
+
 /* ---------------------------------------------------------------------------
  * Batched ensembles of independent WENO5 problems (no reference counterpart:
  * the reference runs one problem per MPI job).  Instances of different grid

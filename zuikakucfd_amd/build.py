@@ -43,7 +43,7 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}",
 ]
 HIP_SOURCES = ["zk_api.hip", "fv_kernels.hip", "dg_kernels.hip", "ens_kernels.hip", "mlp_kernels.hip",
-               "noalloc.hip"]
+               "noalloc.hip", "stream.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -38,7 +38,7 @@ EXPORTED = [
     "zk_mlp_create", "zk_mlp_destroy", "zk_mlp_last_error", "zk_mlp_param_count", "zk_mlp_get_params",
     "zk_mlp_set_params", "zk_mlp_grad", "zk_mlp_infer", "zk_mlp_dataset", "zk_mlp_clear_dataset", "zk_mlp_train",
     "zk_harvest", "zk_attach_mlp", "zk_learned_stats", "zk_mlp_features",
-    "zk_weno_step_noalloc",
+    "zk_weno_step_noalloc", "zk_stream_bandwidth",
 ]
 _NO_STATUS = ("zk_last_error", "zk_ensemble_last_error", "zk_mlp_last_error", "zk_mlp_param_count")
 
@@ -108,6 +108,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.zk_step_timed.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_float)]
     lib.zk_get_flags.argtypes = [vp, C.POINTER(C.c_uint8)]
     lib.zk_updated_cells.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.zk_stream_bandwidth.argtypes = [C.c_int, C.c_size_t, dp, dp]
     lib.zk_profile.argtypes = [vp, C.c_int]
     lib.zk_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.zk_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
@@ -392,6 +393,17 @@ def weno_step_noalloc(Q_ptr: int, bctype_ptr: int, ny: int, nx: int, da: float, 
                                   C.c_void_p(bctype_ptr), float(dt), int(nsteps), C.c_void_p(stream or None))
     if st != ZK_OK:
         _raise(st, f"zk_weno_step_noalloc: {_ERR_NAMES.get(st, st)}: {lib.zk_last_error(None).decode()}")
+
+
+def stream_bandwidth(array_bytes: int, device: int = 0) -> tuple[float, float]:
+    """zk_stream_bandwidth: GB/s of a copy (1 read + 1 write) and of an add
+    (2 reads + 1 write) over arrays of array_bytes on `device`."""
+    lib = load_library()
+    cp, ad = C.c_double(), C.c_double()
+    st = lib.zk_stream_bandwidth(int(device), int(array_bytes), C.byref(cp), C.byref(ad))
+    if st != ZK_OK:
+        _raise(st, f"zk_stream_bandwidth: {_ERR_NAMES.get(st, st)}: {lib.zk_last_error(None).decode()}")
+    return cp.value, ad.value
 
 
 def _f32p(a):
