@@ -65,6 +65,18 @@ def test_documented_errors():
     t.close()
 
 
+def test_stream_bandwidth():
+    """zk_stream_bandwidth: positive rates below the HBM peak for 256 MiB
+    arrays, and its argument errors."""
+    say("stream bandwidth reference")
+    copy, add = zk.stream_bandwidth(256 << 20)
+    assert 500.0 < copy < 8000.0 and 500.0 < add < 8000.0, (copy, add)
+    with pytest.raises(zk.ZkError, match="4 KiB"):
+        zk.stream_bandwidth(1024)
+    with pytest.raises(zk.ZkError, match="device ordinal"):
+        zk.stream_bandwidth(1 << 20, device=99)
+
+
 @pytest.mark.parametrize("center", [0, -1])
 def test_weno_degenerate_grids(center):
     """7x7: one updated cell, or none (the centre a ghost): only the stage
